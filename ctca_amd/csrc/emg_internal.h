@@ -1,0 +1,177 @@
+// emg_internal.h -- context layout and kernel launchers of libemgpu.
+// Not part of the public ABI (include/emgpu.h is).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "emgpu.h"
+
+namespace emg {
+
+constexpr int kBlock = 256;
+constexpr int kFixShift = 24;  // rho fixed point: unit CIC weight = 2^24
+constexpr double kFixScale = 16777216.0;
+constexpr int kJShift = 32;  // current fixed point: 2^32 per (cell / unit time)
+constexpr double kJScale = 4294967296.0;
+constexpr int kCellBits = 10;  // packed cell index: ix | iy << 10 | iz << 20
+constexpr int kCellMask = (1 << kCellBits) - 1;
+
+// device error flags
+constexpr int kFlagDomain = 1;    // particle outside [0, L] / non-finite
+constexpr int kFlagSpillLost = 2; // FP32 re-bucketing lost a particle (capacity)
+constexpr int kFlagFar = 4;       // particle moved more than one box length
+
+enum TimerKind { kTPush = 0, kTDeposit = 1, kTSort = 2, kTRelocate = 3, kTEnergy = 4, kTNum = 5 };
+
+struct Grid {
+  int nx, ny, nz;
+  double slx, sly, slz;
+};
+
+struct SpeciesTab {
+  int nspec;
+  int64_t off[EMG_MAX_SPECIES + 1];  // segment start in the FP64 state arrays
+  int64_t cnt[EMG_MAX_SPECIES];
+  double q[EMG_MAX_SPECIES];
+  double qm[EMG_MAX_SPECIES];
+};
+
+// FP64 (plain) particle state, structure of arrays, species segments at off[]
+struct State64 {
+  double *x, *y, *z, *vx, *vy, *vz;
+  int32_t *pid;
+};
+
+// FP32 cell + offset state, structure of arrays, tile-bucketed
+struct StateFx {
+  int32_t *cell;  // packed ix | iy<<10 | iz<<20
+  float *fx, *fy, *fz, *vx, *vy, *vz;
+  int32_t *pid;   // nullptr unless the context tracks pid
+};
+
+struct FxBuckets {
+  int tile;            // tile edge T
+  int ntx, nty, ntz;   // tiles per axis
+  int ntiles;          // per species
+  int nb;              // nspec * ntiles
+  int64_t cap_total;   // slots per buffer (bucket area + spill)
+  int64_t area;        // bucket area size (spill region starts here)
+  int64_t spill_cap;
+  int64_t *start[2];   // [nb + 1]
+  int32_t *count[2];   // [nb]
+  int32_t *capb;       // [nb] capacity of the output buckets of the current step
+  int32_t *hist;       // [nb] bucketing scratch: histogram
+  int32_t *cursor;     // [nb] bucketing scratch: scatter cursors
+  int32_t *spill[2];   // [1] spill counters
+  StateFx st[2];
+  int cur;             // buffer holding the live particles
+};
+
+struct Timer {
+  hipEvent_t ev[2];
+  int pending;
+  double total_ms;
+  int64_t launches;
+  double last_ms;
+};
+
+}  // namespace emg
+
+struct emg_ctx {
+  emg_config cfg;
+  int device;
+  hipStream_t stream;
+  bool own_stream;
+  emg::Grid g;
+  int64_t ncell;
+  emg::SpeciesTab sp;
+  int64_t cap_total;  // FP64: sum of capacities
+
+  // FP64 state
+  emg::State64 s64;
+  emg::State64 t64;  // sort scratch (lazy)
+  bool have_particles;
+
+  // FP32 state
+  emg::FxBuckets fx;
+
+  // fields
+  double *eb_dev;
+  size_t eb_dev_bytes;
+  double *node64;  // (nx+1)(ny+1)(nz+1) x 8 doubles, nodes 0..n
+  float *node32;   // nx ny nz x 8 floats, periodic nodes
+  bool fields_set;
+
+  // deposits
+  double *rho64;               // nspec x ncell, q-weighted (FP64 form)
+  unsigned long long *rhofx;   // nspec x ncell, unit-weight fixed point
+  double *j64;                 // nspec x 3 x ncell, unscaled zigzag sums
+  unsigned long long *jfx;     // nspec x 3 x ncell fixed point
+  bool rho_valid;
+  bool j_valid;
+
+  // energy
+  double *ke_part;  // partial sums
+  int64_t ke_part_n;
+  double *ke_out;   // [nspec * 3]
+  bool ke_valid;
+
+  // sort scratch
+  int32_t *sort_count;   // nspec x ncell (or nb)
+  int32_t *sort_cursor;
+  int64_t *sort_start;
+  int64_t sort_len;
+  int64_t *scan_tmp;
+  int64_t scan_tmp_len;
+
+  int *dflag;       // device error flags
+  int *hflag;       // pinned host mirror
+
+  bool timing;
+  emg::Timer timers[emg::kTNum];
+};
+
+namespace emg {
+
+// error plumbing (emg_api.hip)
+int set_error(int code, const std::string &msg);
+int hip_check(hipError_t e, const char *what);
+
+// timers
+void timer_begin(emg_ctx *c, int kind);
+void timer_end(emg_ctx *c, int kind);
+
+// scan: exclusive prefix sum of n int32 values into int64 out[0..n]; tmp >= scan_tmp_need(n)
+int64_t scan_tmp_need(int64_t n);
+void exclusive_scan_i32(const int32_t *in, int64_t *out, int64_t n, int64_t *tmp, hipStream_t s);
+
+// kernels: FP64 mirror (k_fp64.hip)
+void launch_relocate64(emg_ctx *c, int elo[3], int eext[3]);
+void launch_positn64(emg_ctx *c, int ustep);
+void launch_push_es64(emg_ctx *c, int ustep);
+void launch_deposit64(emg_ctx *c);
+void launch_push_em64(emg_ctx *c, int ustep);
+
+// common kernels (k_common.hip)
+void launch_relocate32(emg_ctx *c);  // node64-independent: from eb_dev
+void launch_energy64(emg_ctx *c);
+void launch_read_rho(emg_ctx *c, int species, double *dst_dev, const int lo[3], const int hi[3]);
+void launch_read_current(emg_ctx *c, double *dst_dev, const int lo[3], const int hi[3]);
+void launch_zero(void *p, size_t bytes, hipStream_t s);
+int sort64(emg_ctx *c);
+int init_device(emg_ctx *c, const emg_device_species *spec, int nspec, uint64_t seed);
+
+// FP32 + fixed point (k_fx.hip)
+int fx_alloc(emg_ctx *c, int64_t ntotal);
+int fx_from64(emg_ctx *c, const State64 &src);       // convert + bucket
+int fx_to64(emg_ctx *c, State64 &dst);               // convert back (species contiguous)
+void launch_fx_step(emg_ctx *c, int ustep, bool push, bool deposit, bool em);
+void launch_fx_deposit(emg_ctx *c);
+void launch_fx_energy(emg_ctx *c);
+void fx_energy_from_push(emg_ctx *c);
+int fx_rebucket(emg_ctx *c);
+
+}  // namespace emg

@@ -1,0 +1,427 @@
+// k_common.hip -- kernels shared by both precision forms: fills, the
+// exclusive scan used by the sorts, deterministic energy reductions, grid
+// read-out into the reference's Fortran halo layouts, the FP64 cell sort and
+// the seeded device initialiser.
+#include <cmath>
+
+#include "emg_internal.h"
+
+namespace emg {
+namespace {
+
+__global__ void k_zero16(uint4 *p, int64_t n16) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+__global__ void k_zero4(uint32_t *p, int64_t n4) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
+// ---------------------------------------------------------------- scan
+constexpr int kScanPer = 16;
+constexpr int kScanChunk = kBlock * kScanPer;
+
+__device__ int64_t block_excl_scan(int64_t v, int64_t *total) {
+  __shared__ int64_t sh[kBlock];
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = 1; o < kBlock; o <<= 1) {
+    int64_t t = threadIdx.x >= (unsigned)o ? sh[threadIdx.x - o] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += t;
+    __syncthreads();
+  }
+  int64_t incl = sh[threadIdx.x];
+  *total = sh[kBlock - 1];
+  __syncthreads();
+  return incl - v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_sums(const int32_t *in, int64_t n, int64_t *bsum) {
+  int64_t base = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * kScanPer;
+  int64_t s = 0;
+  for (int k = 0; k < kScanPer; k++)
+    if (base + k < n) s += in[base + k];
+  int64_t tot;
+  block_excl_scan(s, &tot);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_top(int64_t *bsum, int64_t nblk) {
+  // exclusive scan of the block sums in place, bsum[nblk] = total
+  int64_t carry = 0;
+  for (int64_t base = 0; base < nblk; base += kBlock) {
+    int64_t i = base + threadIdx.x;
+    int64_t v = i < nblk ? bsum[i] : 0;
+    int64_t tot;
+    int64_t ex = block_excl_scan(v, &tot);
+    if (i < nblk) bsum[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) bsum[nblk] = carry;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_final(const int32_t *in, int64_t n,
+                                                       const int64_t *bsum, int64_t nblk,
+                                                       int64_t *out) {
+  int64_t base = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * kScanPer;
+  int64_t s = 0;
+  for (int k = 0; k < kScanPer; k++)
+    if (base + k < n) s += in[base + k];
+  int64_t tot;
+  int64_t run = block_excl_scan(s, &tot) + bsum[blockIdx.x];
+  for (int k = 0; k < kScanPer; k++)
+    if (base + k < n) {
+      out[base + k] = run;
+      run += in[base + k];
+    }
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = bsum[nblk];
+}
+
+// ---------------------------------------------------------------- energy
+constexpr int kEnergyBlocks = 512;
+
+__global__ __launch_bounds__(kBlock) void k_energy64_part(State64 s, SpeciesTab sp, double *part) {
+  int is = blockIdx.y;
+  double a[3] = {0.0, 0.0, 0.0};
+  for (int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x; m < sp.cnt[is];
+       m += (int64_t)gridDim.x * kBlock) {
+    int64_t p = sp.off[is] + m;
+    double vx = s.vx[p], vy = s.vy[p], vz = s.vz[p];
+    a[0] = a[0] + vx * vx;
+    a[1] = a[1] + vy * vy;
+    a[2] = a[2] + vz * vz;
+  }
+  __shared__ double red[3][kBlock];
+  for (int d = 0; d < 3; d++) red[d][threadIdx.x] = a[d];
+  __syncthreads();
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < (unsigned)o)
+      for (int d = 0; d < 3; d++) red[d][threadIdx.x] += red[d][threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3)
+    part[3 * ((int64_t)is * gridDim.x + blockIdx.x) + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(kBlock) void k_energy64_final(const double *part, int nblk, double *out) {
+  int is = blockIdx.x;
+  __shared__ double red[3][kBlock];
+  double a[3] = {0.0, 0.0, 0.0};
+  for (int k = threadIdx.x; k < nblk; k += kBlock)
+    for (int d = 0; d < 3; d++) a[d] += part[3 * ((int64_t)is * nblk + k) + d];
+  for (int d = 0; d < 3; d++) red[d][threadIdx.x] = a[d];
+  __syncthreads();
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < (unsigned)o)
+      for (int d = 0; d < 3; d++) red[d][threadIdx.x] += red[d][threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) out[3 * is + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// ---------------------------------------------------------------- read-out
+__device__ __forceinline__ int pmod(int i, int n) {
+  int r = i % n;
+  return r < 0 ? r + n : r;
+}
+
+struct ReadArgs {
+  Grid g;
+  int nspec;
+  int species;  // -1: all species
+  double q[EMG_MAX_SPECIES];
+  int lo[3], ext[3];
+  int64_t ncell;
+};
+
+__global__ __launch_bounds__(kBlock) void k_read_rho64(ReadArgs r, const double *rho, double *out) {
+  int64_t n = (int64_t)r.ext[0] * r.ext[1] * r.ext[2];
+  int64_t o = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (o >= n) return;
+  int i = (int)(o % r.ext[0]) + r.lo[0];
+  int j = (int)((o / r.ext[0]) % r.ext[1]) + r.lo[1];
+  int k = (int)(o / ((int64_t)r.ext[0] * r.ext[1])) + r.lo[2];
+  int64_t c = ((int64_t)pmod(k, r.g.nz) * r.g.ny + pmod(j, r.g.ny)) * r.g.nx + pmod(i, r.g.nx);
+  double v = 0.0;
+  if (r.species >= 0) {
+    v = rho[(int64_t)r.species * r.ncell + c];
+  } else {
+    for (int s = 0; s < r.nspec; s++) v = v + rho[(int64_t)s * r.ncell + c];
+  }
+  out[o] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_read_rhofx(ReadArgs r, const unsigned long long *rho,
+                                                      double *out) {
+  int64_t n = (int64_t)r.ext[0] * r.ext[1] * r.ext[2];
+  int64_t o = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (o >= n) return;
+  int i = (int)(o % r.ext[0]) + r.lo[0];
+  int j = (int)((o / r.ext[0]) % r.ext[1]) + r.lo[1];
+  int k = (int)(o / ((int64_t)r.ext[0] * r.ext[1])) + r.lo[2];
+  int64_t c = ((int64_t)pmod(k, r.g.nz) * r.g.ny + pmod(j, r.g.ny)) * r.g.nx + pmod(i, r.g.nx);
+  double v = 0.0;
+  for (int s = 0; s < r.nspec; s++) {
+    if (r.species >= 0 && s != r.species) continue;
+    long long w = (long long)rho[(int64_t)s * r.ncell + c];
+    v = v + r.q[s] * ((double)w * (1.0 / kFixScale));
+  }
+  out[o] = v;
+}
+
+// aj(3, lo:hi): sum over species of q(is) * J(is) in species order
+template <bool FIX>
+__global__ __launch_bounds__(kBlock) void k_read_current(ReadArgs r, const void *jacc, double *out) {
+  int64_t n = (int64_t)r.ext[0] * r.ext[1] * r.ext[2];
+  int64_t o = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (o >= n) return;
+  int i = (int)(o % r.ext[0]) + r.lo[0];
+  int j = (int)((o / r.ext[0]) % r.ext[1]) + r.lo[1];
+  int k = (int)(o / ((int64_t)r.ext[0] * r.ext[1])) + r.lo[2];
+  int64_t c = ((int64_t)pmod(k, r.g.nz) * r.g.ny + pmod(j, r.g.ny)) * r.g.nx + pmod(i, r.g.nx);
+  for (int d = 0; d < 3; d++) {
+    double v = 0.0;
+    for (int s = 0; s < r.nspec; s++) {
+      int64_t q = ((int64_t)s * 3 + d) * r.ncell + c;
+      double js = FIX ? (double)(long long)((const unsigned long long *)jacc)[q] * (1.0 / kJScale)
+                      : ((const double *)jacc)[q];
+      v = v + js * r.q[s];
+    }
+    out[3 * o + d] = v;
+  }
+}
+
+// ---------------------------------------------------------------- FP64 sort
+__device__ __forceinline__ int64_t cell_key(double x, double y, double z, const Grid &g) {
+  int i = (int)x, j = (int)y, k = (int)z;
+  i = i < 0 ? 0 : (i >= g.nx ? g.nx - 1 : i);
+  j = j < 0 ? 0 : (j >= g.ny ? g.ny - 1 : j);
+  k = k < 0 ? 0 : (k >= g.nz ? g.nz - 1 : k);
+  return ((int64_t)k * g.ny + j) * g.nx + i;
+}
+
+__global__ __launch_bounds__(kBlock) void k_sort_hist(State64 s, SpeciesTab sp, Grid g, int64_t ncell,
+                                                      int32_t *count) {
+  int is = blockIdx.y;
+  int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (m >= sp.cnt[is]) return;
+  m += sp.off[is];
+  atomicAdd(&count[(int64_t)is * ncell + cell_key(s.x[m], s.y[m], s.z[m], g)], 1);
+}
+
+__global__ __launch_bounds__(kBlock) void k_sort_scatter(State64 s, State64 t, SpeciesTab sp, Grid g,
+                                                         int64_t ncell, const int64_t *start,
+                                                         int32_t *cursor) {
+  int is = blockIdx.y;
+  int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (m >= sp.cnt[is]) return;
+  m += sp.off[is];
+  int64_t key = (int64_t)is * ncell + cell_key(s.x[m], s.y[m], s.z[m], g);
+  int64_t d = sp.off[is] + (start[key] - start[(int64_t)is * ncell]) + atomicAdd(&cursor[key], 1);
+  t.x[d] = s.x[m];
+  t.y[d] = s.y[m];
+  t.z[d] = s.z[m];
+  t.vx[d] = s.vx[m];
+  t.vy[d] = s.vy[m];
+  t.vz[d] = s.vz[m];
+  t.pid[d] = s.pid[m];
+}
+
+// ---------------------------------------------------------------- device init
+// SplitMix64 (Steele, Lea, Flood 2014) over a (species, particle, draw)
+// counter: a stateless stream, so any particle can be generated independently.
+__device__ __forceinline__ uint64_t splitmix(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ double u01(uint64_t seed, uint64_t ctr) {
+  uint64_t z = splitmix(seed + ctr * 0x9e3779b97f4a7c15ull);
+  return ((double)(z >> 11) + 0.5) * (1.0 / 9007199254740992.0);  // (0, 1)
+}
+
+struct InitSpec {
+  int64_t count;
+  double vperp, vpara, drift[3], t[9], ramp;
+};
+
+__global__ __launch_bounds__(kBlock) void k_init(State64 st, int64_t off, int is, InitSpec p,
+                                                 Grid g, uint64_t seed) {
+  int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (m >= p.count) return;
+  uint64_t base = (((uint64_t)is << 40) + (uint64_t)m) * 8ull;
+  double u[7];
+  for (int k = 0; k < 7; k++) u[k] = u01(seed, base + k);
+  double x = g.slx * u[0], y = g.sly * u[1], z;
+  if (p.ramp > 1.0)
+    z = g.slz * log1p(u[2] * (p.ramp - 1.0)) / log(p.ramp);
+  else
+    z = g.slz * u[2];
+  if (x >= g.slx) x = 0.0;
+  if (y >= g.sly) y = 0.0;
+  if (z >= g.slz) z = 0.0;
+  double r1 = sqrt(-2.0 * log(u[3])), th = 6.283185307179586 * u[4];
+  double r2 = sqrt(-2.0 * log(u[5])), th2 = 6.283185307179586 * u[6];
+  double vx = p.vperp * r1 * cos(th), vy = p.vperp * r1 * sin(th), vz = p.vpara * r2 * cos(th2);
+  int64_t d = off + m;
+  st.x[d] = x;
+  st.y[d] = y;
+  st.z[d] = z;
+  st.vx[d] = p.t[0] * vx + p.t[1] * vy + p.t[2] * vz + p.drift[0];
+  st.vy[d] = p.t[3] * vx + p.t[4] * vy + p.t[5] * vz + p.drift[1];
+  st.vz[d] = p.t[6] * vx + p.t[7] * vy + p.t[8] * vz + p.drift[2];
+  st.pid[d] = 0;
+}
+
+dim3 species_grid(const SpeciesTab &sp) {
+  int64_t mx = 1;
+  for (int s = 0; s < sp.nspec; s++) mx = sp.cnt[s] > mx ? sp.cnt[s] : mx;
+  return dim3((unsigned)((mx + kBlock - 1) / kBlock), (unsigned)sp.nspec, 1);
+}
+
+}  // namespace
+
+void launch_zero(void *p, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return;
+  if ((bytes & 15) == 0 && (((uintptr_t)p) & 15) == 0) {
+    int64_t n16 = (int64_t)(bytes / 16);
+    unsigned grid = (unsigned)std::min<int64_t>((n16 + kBlock - 1) / kBlock, 65536);
+    hipLaunchKernelGGL(k_zero16, dim3(grid), dim3(kBlock), 0, s, (uint4 *)p, n16);
+  } else {
+    int64_t n4 = (int64_t)(bytes / 4);
+    unsigned grid = (unsigned)std::min<int64_t>((n4 + kBlock - 1) / kBlock, 65536);
+    hipLaunchKernelGGL(k_zero4, dim3(grid), dim3(kBlock), 0, s, (uint32_t *)p, n4);
+  }
+}
+
+int64_t scan_tmp_need(int64_t n) { return (n + kScanChunk - 1) / kScanChunk + 1; }
+
+void exclusive_scan_i32(const int32_t *in, int64_t *out, int64_t n, int64_t *tmp, hipStream_t s) {
+  int64_t nblk = (n + kScanChunk - 1) / kScanChunk;
+  if (nblk == 0) {
+    launch_zero(out, sizeof(int64_t), s);
+    return;
+  }
+  hipLaunchKernelGGL(k_scan_sums, dim3((unsigned)nblk), dim3(kBlock), 0, s, in, n, tmp);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kBlock), 0, s, tmp, nblk);
+  hipLaunchKernelGGL(k_scan_final, dim3((unsigned)nblk), dim3(kBlock), 0, s, in, n, tmp, nblk, out);
+}
+
+void launch_energy64(emg_ctx *c) {
+  timer_begin(c, kTEnergy);
+  hipLaunchKernelGGL(k_energy64_part, dim3(kEnergyBlocks, c->sp.nspec), dim3(kBlock), 0, c->stream,
+                     c->s64, c->sp, c->ke_part);
+  hipLaunchKernelGGL(k_energy64_final, dim3(c->sp.nspec), dim3(kBlock), 0, c->stream, c->ke_part,
+                     kEnergyBlocks, c->ke_out);
+  timer_end(c, kTEnergy);
+}
+
+int64_t energy64_part_need(int nspec) { return (int64_t)kEnergyBlocks * nspec * 3; }
+
+static ReadArgs read_args(emg_ctx *c, int species, const int lo[3], const int hi[3]) {
+  ReadArgs r;
+  r.g = c->g;
+  r.nspec = c->sp.nspec;
+  r.species = species;
+  for (int s = 0; s < EMG_MAX_SPECIES; s++) r.q[s] = c->cfg.q[s];
+  for (int d = 0; d < 3; d++) {
+    r.lo[d] = lo[d];
+    r.ext[d] = hi[d] - lo[d] + 1;
+  }
+  r.ncell = c->ncell;
+  return r;
+}
+
+void launch_read_rho(emg_ctx *c, int species, double *dst, const int lo[3], const int hi[3]) {
+  ReadArgs r = read_args(c, species, lo, hi);
+  int64_t n = (int64_t)r.ext[0] * r.ext[1] * r.ext[2];
+  unsigned grid = (unsigned)((n + kBlock - 1) / kBlock);
+  if (c->cfg.precision == EMG_PLAIN_FP64)
+    hipLaunchKernelGGL(k_read_rho64, dim3(grid), dim3(kBlock), 0, c->stream, r, c->rho64, dst);
+  else
+    hipLaunchKernelGGL(k_read_rhofx, dim3(grid), dim3(kBlock), 0, c->stream, r, c->rhofx, dst);
+}
+
+void launch_read_current(emg_ctx *c, double *dst, const int lo[3], const int hi[3]) {
+  ReadArgs r = read_args(c, -1, lo, hi);
+  int64_t n = (int64_t)r.ext[0] * r.ext[1] * r.ext[2];
+  unsigned grid = (unsigned)((n + kBlock - 1) / kBlock);
+  if (c->cfg.precision == EMG_PLAIN_FP64)
+    hipLaunchKernelGGL(k_read_current<false>, dim3(grid), dim3(kBlock), 0, c->stream, r,
+                       (const void *)c->j64, dst);
+  else
+    hipLaunchKernelGGL(k_read_current<true>, dim3(grid), dim3(kBlock), 0, c->stream, r,
+                       (const void *)c->jfx, dst);
+}
+
+int sort64(emg_ctx *c) {
+  int64_t len = c->ncell * c->sp.nspec;
+  if (c->sort_len < len) {
+    if (c->sort_count) (void)hipFree(c->sort_count);
+    if (c->sort_cursor) (void)hipFree(c->sort_cursor);
+    if (c->sort_start) (void)hipFree(c->sort_start);
+    c->sort_count = nullptr;
+    c->sort_cursor = nullptr;
+    c->sort_start = nullptr;
+    c->sort_len = 0;
+    if (hipMalloc(&c->sort_count, sizeof(int32_t) * len) != hipSuccess ||
+        hipMalloc(&c->sort_cursor, sizeof(int32_t) * len) != hipSuccess ||
+        hipMalloc(&c->sort_start, sizeof(int64_t) * (len + 1)) != hipSuccess)
+      return set_error(EMG_ERR_NOMEM, "sort scratch");
+    c->sort_len = len;
+  }
+  int64_t need = scan_tmp_need(len);
+  if (need > c->scan_tmp_len) {
+    if (c->scan_tmp) (void)hipFree(c->scan_tmp);
+    if (hipMalloc(&c->scan_tmp, sizeof(int64_t) * need) != hipSuccess)
+      return set_error(EMG_ERR_NOMEM, "scan scratch");
+    c->scan_tmp_len = need;
+  }
+  if (!c->t64.x) {
+    void *p[7];
+    for (int q = 0; q < 7; q++)
+      if (hipMalloc(&p[q], (q < 6 ? sizeof(double) : sizeof(int32_t)) * c->cap_total) != hipSuccess)
+        return set_error(EMG_ERR_NOMEM, "sort buffers");
+    c->t64 = State64{(double *)p[0], (double *)p[1], (double *)p[2], (double *)p[3],
+                     (double *)p[4], (double *)p[5], (int32_t *)p[6]};
+  }
+  launch_zero(c->sort_count, sizeof(int32_t) * len, c->stream);
+  launch_zero(c->sort_cursor, sizeof(int32_t) * len, c->stream);
+  dim3 grid = species_grid(c->sp);
+  timer_begin(c, kTSort);
+  hipLaunchKernelGGL(k_sort_hist, grid, dim3(kBlock), 0, c->stream, c->s64, c->sp, c->g, c->ncell,
+                     c->sort_count);
+  exclusive_scan_i32(c->sort_count, c->sort_start, len, c->scan_tmp, c->stream);
+  hipLaunchKernelGGL(k_sort_scatter, grid, dim3(kBlock), 0, c->stream, c->s64, c->t64, c->sp, c->g,
+                     c->ncell, c->sort_start, c->sort_cursor);
+  timer_end(c, kTSort);
+  State64 tmp = c->s64;
+  c->s64 = c->t64;
+  c->t64 = tmp;
+  return hip_check(hipGetLastError(), "sort");
+}
+
+int init_device_into(emg_ctx *c, State64 &st, const emg_device_species *spec, int nspec,
+                     uint64_t seed) {
+  for (int s = 0; s < nspec; s++) {
+    InitSpec p;
+    p.count = spec[s].count;
+    p.vperp = spec[s].vth_perp;
+    p.vpara = spec[s].vth_para;
+    for (int d = 0; d < 3; d++) p.drift[d] = spec[s].drift[d];
+    for (int d = 0; d < 9; d++) p.t[d] = spec[s].t[d];
+    p.ramp = spec[s].ramp_ratio;
+    if (p.count > 0)
+      hipLaunchKernelGGL(k_init, dim3((unsigned)((p.count + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                         c->stream, st, c->sp.off[s], s, p, c->g, seed);
+  }
+  return hip_check(hipGetLastError(), "init_device");
+}
+
+}  // namespace emg

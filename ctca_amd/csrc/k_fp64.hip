@@ -1,0 +1,403 @@
+// k_fp64.hip -- plain FP64 kernels of the particle path: the drop-in form that
+// mirrors the reference arithmetic operation by operation.  Compiled with
+// -ffp-contract=off, so every multiply and add is rounded separately in the
+// reference's source order (and so bit-identical to a CPU restatement built the
+// same way).  One thread per particle, species segments on blockIdx.y, state
+// in place (structure of arrays), deposition with FP64 global atomics into
+// periodic accumulators (node index mod n == the single-rank fold).
+#include "emg_internal.h"
+
+namespace emg {
+namespace {
+
+__device__ __forceinline__ int64_t nidx64(const Grid &g, int i, int j, int k) {
+  // nodal FP64 field, nodes 0..n per axis
+  return ((int64_t)k * (g.ny + 1) + j) * (g.nx + 1) + i;
+}
+
+__device__ __forceinline__ int wrapn(int i, int n) {
+  // node / cell index in [-n, 2n) -> [0, n)
+  return i < 0 ? i + n : (i >= n ? i - n : i);
+}
+
+__device__ __forceinline__ int64_t cidx(const Grid &g, int i, int j, int k) {
+  return ((int64_t)wrapn(k, g.nz) * g.ny + wrapn(j, g.ny)) * g.nx + wrapn(i, g.nx);
+}
+
+// Guard against positions the reference would index out of bounds with:
+// anything outside [0, L] or non-finite is clamped and reported.
+__device__ __forceinline__ void domain_guard(double &x, double &y, double &z, const Grid &g,
+                                             int *flag) {
+  bool ok = (x >= 0.0) && (x <= g.slx) && (y >= 0.0) && (y <= g.sly) && (z >= 0.0) &&
+            (z <= g.slz);
+  if (!ok) {
+    atomicOr(flag, kFlagDomain);
+    x = (x >= 0.0) ? (x <= g.slx ? x : g.slx) : 0.0;
+    y = (y >= 0.0) ? (y <= g.sly ? y : g.sly) : 0.0;
+    z = (z >= 0.0) ? (z <= g.slz ? z : g.slz) : 0.0;
+  }
+}
+
+struct Cic {
+  double v1, v2, v3, v4, v5, v6, v7, v8;
+  int i, j, k;
+};
+
+// CIC weights, src/pcl2field/psolve2.F90:154-183 (charge.F90:160-193).
+// xl, yl, zl are the cell-local coordinates, i/j/k the already truncated cell.
+__device__ __forceinline__ Cic cic(double xl, double yl, double zl, int i, int j, int k) {
+  Cic w;
+  w.i = i;
+  w.j = j;
+  w.k = k;
+  double x1 = xl - (double)i, y1 = yl - (double)j, z1 = zl - (double)k;
+  double xy1 = x1 * y1, xz1 = x1 * z1, yz1 = y1 * z1;
+  double z2 = 1.0 - z1;
+  double xz2 = x1 * z2, yz2 = y1 * z2;
+  w.v3 = xy1 * z1;
+  w.v2 = xz1 - w.v3;
+  w.v4 = yz1 - w.v3;
+  w.v1 = z1 - xz1 - w.v4;
+  w.v7 = xy1 * z2;
+  w.v6 = xz2 - w.v7;
+  w.v8 = yz2 - w.v7;
+  w.v5 = z2 - xz2 - w.v8;
+  return w;
+}
+
+// psolve2.F90:195-218: eight-node gather of the six nodal components in the
+// reference's node order v1..v8 and summation order.
+__device__ __forceinline__ void gather6(const double *__restrict__ node, const Grid &g,
+                                        const Cic &w, double f[6]) {
+  int i1 = min(w.i + 1, g.nx), j1 = min(w.j + 1, g.ny), k1 = min(w.k + 1, g.nz);
+  const double *p1 = node + 8 * nidx64(g, w.i, w.j, k1);
+  const double *p2 = node + 8 * nidx64(g, i1, w.j, k1);
+  const double *p3 = node + 8 * nidx64(g, i1, j1, k1);
+  const double *p4 = node + 8 * nidx64(g, w.i, j1, k1);
+  const double *p5 = node + 8 * nidx64(g, w.i, w.j, w.k);
+  const double *p6 = node + 8 * nidx64(g, i1, w.j, w.k);
+  const double *p7 = node + 8 * nidx64(g, i1, j1, w.k);
+  const double *p8 = node + 8 * nidx64(g, w.i, j1, w.k);
+#pragma unroll
+  for (int c = 0; c < 6; c++)
+    f[c] = p1[c] * w.v1 + p2[c] * w.v2 + p3[c] * w.v3 + p4[c] * w.v4 + p5[c] * w.v5 +
+           p6[c] * w.v6 + p7[c] * w.v7 + p8[c] * w.v8;
+}
+
+// Buneman-Boris, psolve2.F90:233-257 (ewmodel = 0: the wave field is +0.0).
+__device__ __forceinline__ void boris64(double &vx, double &vy, double &vz, const double f[6],
+                                        double qmp) {
+  const double ewx = 0.0, ewy = 0.0;
+  double eex = (f[0] + ewx) * qmp;
+  double eey = (f[1] + ewy) * qmp;
+  double eez = f[2] * qmp;
+  double bbx = f[3] * qmp, bby = f[4] * qmp, bbz = f[5] * qmp;
+  double bor = 2.0 / (1.0 + bbx * bbx + bby * bby + bbz * bbz);
+  vx = vx + eex;
+  vy = vy + eey;
+  vz = vz + eez;
+  double vxt = vx + vy * bbz - vz * bby;
+  double vyt = vy + vz * bbx - vx * bbz;
+  double vzt = vz + vx * bby - vy * bbx;
+  vx = vx + bor * (vyt * bbz - vzt * bby);
+  vy = vy + bor * (vzt * bbx - vxt * bbz);
+  vz = vz + bor * (vxt * bby - vyt * bbx);
+  vx = vx + eex;
+  vy = vy + eey;
+  vz = vz + eez;
+}
+
+// psolve2.F90:354-432 / positn.F90:172-254, npbnd = 0: one wrap per axis
+__device__ __forceinline__ void wrap_periodic(double &x, double &y, double &z, const Grid &g,
+                                              int *flag) {
+  if (x < 0.0 || x >= g.slx || y < 0.0 || y >= g.sly || z < 0.0 || z >= g.slz) {
+    if (x < 0.0) x = x + g.slx; else if (x >= g.slx) x = x - g.slx;
+    if (y < 0.0) y = y + g.sly; else if (y >= g.sly) y = y - g.sly;
+    if (z < 0.0) z = z + g.slz; else if (z >= g.slz) z = z - g.slz;
+    if (!(x >= 0.0 && x <= g.slx && y >= 0.0 && y <= g.sly && z >= 0.0 && z <= g.slz))
+      atomicOr(flag, kFlagFar);
+  }
+}
+
+// ---------------------------------------------------------------- grdcrt
+// grdcrt.F90:158-169 (E) and 216-230 (B); writes nodes 0..n (FP64) and, when
+// node32 != nullptr, the periodic FP32 copy (nodes 0..n-1) of the same values.
+__global__ __launch_bounds__(kBlock) void k_relocate(const double *__restrict__ eb, int elx,
+                                                     int ely, int elz, int exx, int exy,
+                                                     double *__restrict__ node64,
+                                                     float *__restrict__ node32, Grid g,
+                                                     double e0x, double e0y, double e0z,
+                                                     double b0x, double b0y, double b0z) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t nn = (int64_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
+  if (idx >= nn) return;
+  int i = (int)(idx % (g.nx + 1));
+  int j = (int)((idx / (g.nx + 1)) % (g.ny + 1));
+  int k = (int)(idx / ((int64_t)(g.nx + 1) * (g.ny + 1)));
+  auto E = [&](int c, int ii, int jj, int kk) -> double {
+    return eb[(((int64_t)(kk - elz) * exy + (jj - ely)) * exx + (ii - elx)) * 6 + c];
+  };
+  double o[6];
+  o[0] = 0.5 * (E(0, i, j, k) + E(0, i - 1, j, k)) + e0x;
+  o[1] = 0.5 * (E(1, i, j, k) + E(1, i, j - 1, k)) + e0y;
+  o[2] = 0.5 * (E(2, i, j, k) + E(2, i, j, k - 1)) + e0z;
+  o[3] = 0.25 * (E(3, i, j, k) + E(3, i, j - 1, k) + E(3, i, j, k - 1) + E(3, i, j - 1, k - 1)) + b0x;
+  o[4] = 0.25 * (E(4, i, j, k) + E(4, i - 1, j, k) + E(4, i, j, k - 1) + E(4, i - 1, j, k - 1)) + b0y;
+  o[5] = 0.25 * (E(5, i, j, k) + E(5, i - 1, j, k) + E(5, i, j - 1, k) + E(5, i - 1, j - 1, k)) + b0z;
+  if (node64) {
+    double *d = node64 + 8 * idx;
+#pragma unroll
+    for (int c = 0; c < 6; c++) d[c] = o[c];
+    d[6] = 0.0;
+    d[7] = 0.0;
+  }
+  if (node32 && i < g.nx && j < g.ny && k < g.nz) {
+    float4 *d = reinterpret_cast<float4 *>(node32 + 8 * (((int64_t)k * g.ny + j) * g.nx + i));
+    d[0] = make_float4((float)o[0], (float)o[1], (float)o[2], (float)o[3]);
+    d[1] = make_float4((float)o[4], (float)o[5], 0.f, 0.f);
+  }
+}
+
+// ---------------------------------------------------------------- positn
+__global__ __launch_bounds__(kBlock) void k_positn64(State64 s, SpeciesTab sp, Grid g, int ustep,
+                                                     int *flag) {
+  int is = blockIdx.y;
+  int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= sp.cnt[is]) return;
+  m += sp.off[is];
+  double us = (double)ustep;
+  double x = s.x[m], y = s.y[m], z = s.z[m];
+  x = x + s.vx[m] * us;
+  y = y + s.vy[m] * us;
+  z = z + s.vz[m] * us;
+  wrap_periodic(x, y, z, g, flag);
+  s.x[m] = x;
+  s.y[m] = y;
+  s.z[m] = z;
+}
+
+// ---------------------------------------------------------------- psolve2
+__global__ __launch_bounds__(kBlock) void k_push_es64(State64 s, SpeciesTab sp,
+                                                      const double *__restrict__ node, Grid g,
+                                                      int ustep, int *flag) {
+  int is = blockIdx.y;
+  int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= sp.cnt[is]) return;
+  m += sp.off[is];
+  double gustep = (double)ustep;
+  double qmp = sp.qm[is] * gustep * 0.5;
+  double x = s.x[m], y = s.y[m], z = s.z[m];
+  double vx = s.vx[m], vy = s.vy[m], vz = s.vz[m];
+  domain_guard(x, y, z, g, flag);
+  int i = (int)x, j = (int)y, k = (int)z;  // int(), psolve2.F90:158-160
+  Cic w = cic(x, y, z, i, j, k);
+  double f[6];
+  gather6(node, g, w, f);
+  boris64(vx, vy, vz, f, qmp);
+  double xmove = vx * gustep, ymove = vy * gustep, zmove = vz * gustep;
+  x = x + xmove;
+  y = y + ymove;
+  z = z + zmove;
+  wrap_periodic(x, y, z, g, flag);
+  s.x[m] = x;
+  s.y[m] = y;
+  s.z[m] = z;
+  s.vx[m] = vx;
+  s.vy[m] = vy;
+  s.vz[m] = vz;
+}
+
+// ---------------------------------------------------------------- charge
+// charge.F90:59-83 + distribute_charge 197-210; rho[is][node mod n] += v_k q(is)
+__global__ __launch_bounds__(kBlock) void k_deposit64(State64 s, SpeciesTab sp, Grid g,
+                                                      double *__restrict__ rho, int64_t ncell,
+                                                      int *flag) {
+  int is = blockIdx.y;
+  int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= sp.cnt[is]) return;
+  m += sp.off[is];
+  double x = s.x[m], y = s.y[m], z = s.z[m];
+  domain_guard(x, y, z, g, flag);
+  int i = (int)x, j = (int)y, k = (int)z;
+  Cic w = cic(x, y, z, i, j, k);
+  double qc = sp.q[is];
+  double *r = rho + (int64_t)is * ncell;
+  int i1 = i + 1, j1 = j + 1, k1 = k + 1;
+  atomicAdd(r + cidx(g, i, j, k1), w.v1 * qc);
+  atomicAdd(r + cidx(g, i1, j, k1), w.v2 * qc);
+  atomicAdd(r + cidx(g, i1, j1, k1), w.v3 * qc);
+  atomicAdd(r + cidx(g, i, j1, k1), w.v4 * qc);
+  atomicAdd(r + cidx(g, i, j, k), w.v5 * qc);
+  atomicAdd(r + cidx(g, i1, j, k), w.v6 * qc);
+  atomicAdd(r + cidx(g, i1, j1, k), w.v7 * qc);
+  atomicAdd(r + cidx(g, i, j1, k), w.v8 * qc);
+}
+
+// ---------------------------------------------------------------- psolve1
+// psolve1.F90:146-789: push with floor() cells, then the Umeda zigzag current
+// of the two segments b -> relay -> a; J[is][c][cell mod n] += vv * w (q(is)
+// is applied at read-out, psolve1.F90:793-802).
+struct Relay {
+  double la, a, r;
+  int ia;
+};
+
+__device__ __forceinline__ Relay relay_axis(double xnew, double lb, double cb, int ib, int n,
+                                            double sl, bool inside, double &pos) {
+  Relay r;
+  if (!inside && xnew < 0.0) {  // psolve1.F90:440-447
+    r.la = xnew;
+    pos = xnew + sl;
+    r.ia = -1;
+    r.a = -1.0;
+    r.r = 0.0;
+  } else if (!inside && xnew >= sl) {  // :462-469
+    r.la = xnew - 0.0;
+    pos = xnew - sl;
+    r.ia = n;
+    r.a = (double)n;
+    r.r = (double)n;
+  } else {  // :381-415, :469-478
+    r.la = xnew - 0.0;
+    pos = xnew;
+    r.ia = (int)floor(r.la);
+    r.a = (double)r.ia;
+    r.r = (ib == r.ia) ? (lb + r.la) * 0.5 : fmax(cb, r.a);
+  }
+  return r;
+}
+
+__global__ __launch_bounds__(kBlock) void k_push_em64(State64 s, SpeciesTab sp,
+                                                      const double *__restrict__ node, Grid g,
+                                                      int ustep, double *__restrict__ jacc,
+                                                      int64_t ncell, int *flag) {
+  int is = blockIdx.y;
+  int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= sp.cnt[is]) return;
+  m += sp.off[is];
+  double gustep = (double)ustep;
+  double rustep = 1.0 / gustep;
+  double qmp = sp.qm[is] * gustep * 0.5;
+  double x = s.x[m], y = s.y[m], z = s.z[m];
+  double vx = s.vx[m], vy = s.vy[m], vz = s.vz[m];
+  domain_guard(x, y, z, g, flag);
+  double xlb = x, ylb = y, zlb = z;
+  int ib = (int)floor(xlb), jb = (int)floor(ylb), kb = (int)floor(zlb);  // :170-172
+  double xb = ib, yb = jb, zb = kb;
+  Cic w = cic(xlb, ylb, zlb, ib, jb, kb);
+  double f[6];
+  gather6(node, g, w, f);
+  boris64(vx, vy, vz, f, qmp);
+  double xmove = vx * gustep, ymove = vy * gustep, zmove = vz * gustep;
+  x = x + xmove;
+  y = y + ymove;
+  z = z + zmove;
+  bool inside = !(x < 0.0 || x >= g.slx || y < 0.0 || y >= g.sly || z < 0.0 || z >= g.slz);
+  double px, py, pz;
+  Relay rx = relay_axis(x, xlb, xb, ib, g.nx, g.slx, inside, px);
+  Relay ry = relay_axis(y, ylb, yb, jb, g.ny, g.sly, inside, py);
+  Relay rz = relay_axis(z, zlb, zb, kb, g.nz, g.slz, inside, pz);
+  if (!(px >= 0.0 && px <= g.slx && py >= 0.0 && py <= g.sly && pz >= 0.0 && pz <= g.slz))
+    atomicOr(flag, kFlagFar);
+  s.x[m] = px;
+  s.y[m] = py;
+  s.z[m] = pz;
+  s.vx[m] = vx;
+  s.vy[m] = vy;
+  s.vz[m] = vz;
+  // :717-757
+  double xd1 = (rx.r + xlb) * 0.5 - xb, yd1 = (ry.r + ylb) * 0.5 - yb, zd1 = (rz.r + zlb) * 0.5 - zb;
+  double vvx1 = (rx.r - xlb) * rustep, vvy1 = (ry.r - ylb) * rustep, vvz1 = (rz.r - zlb) * rustep;
+  double vx1w4 = yd1 * zd1, vx1w2 = yd1 - vx1w4, vx1w3 = zd1 - vx1w4, vx1w1 = 1.0 - yd1 - vx1w3;
+  double vy1w4 = xd1 * zd1, vy1w2 = xd1 - vy1w4, vy1w3 = zd1 - vy1w4, vy1w1 = 1.0 - xd1 - vy1w3;
+  double vz1w4 = xd1 * yd1, vz1w2 = xd1 - vz1w4, vz1w3 = yd1 - vz1w4, vz1w1 = 1.0 - xd1 - vz1w3;
+  double xd2 = (rx.la + rx.r) * 0.5 - rx.a, yd2 = (ry.la + ry.r) * 0.5 - ry.a,
+         zd2 = (rz.la + rz.r) * 0.5 - rz.a;
+  double vvx2 = (rx.la - rx.r) * rustep, vvy2 = (ry.la - ry.r) * rustep,
+         vvz2 = (rz.la - rz.r) * rustep;
+  double vx2w4 = yd2 * zd2, vx2w2 = yd2 - vx2w4, vx2w3 = zd2 - vx2w4, vx2w1 = 1.0 - yd2 - vx2w3;
+  double vy2w4 = xd2 * zd2, vy2w2 = xd2 - vy2w4, vy2w3 = zd2 - vy2w4, vy2w1 = 1.0 - xd2 - vy2w3;
+  double vz2w4 = xd2 * yd2, vz2w2 = xd2 - vz2w4, vz2w3 = yd2 - vz2w4, vz2w1 = 1.0 - xd2 - vz2w3;
+  double *jx = jacc + ((int64_t)is * 3 + 0) * ncell;
+  double *jy = jacc + ((int64_t)is * 3 + 1) * ncell;
+  double *jz = jacc + ((int64_t)is * 3 + 2) * ncell;
+  int i1b = ib + 1, j1b = jb + 1, k1b = kb + 1;
+  int ia = rx.ia, ja = ry.ia, ka = rz.ia, i1a = ia + 1, j1a = ja + 1, k1a = ka + 1;
+  // :759-789
+  atomicAdd(jx + cidx(g, ib, jb, kb), vvx1 * vx1w1);
+  atomicAdd(jy + cidx(g, ib, jb, kb), vvy1 * vy1w1);
+  atomicAdd(jz + cidx(g, ib, jb, kb), vvz1 * vz1w1);
+  atomicAdd(jx + cidx(g, ib, j1b, kb), vvx1 * vx1w2);
+  atomicAdd(jy + cidx(g, i1b, jb, kb), vvy1 * vy1w2);
+  atomicAdd(jz + cidx(g, i1b, jb, kb), vvz1 * vz1w2);
+  atomicAdd(jx + cidx(g, ib, jb, k1b), vvx1 * vx1w3);
+  atomicAdd(jy + cidx(g, ib, jb, k1b), vvy1 * vy1w3);
+  atomicAdd(jz + cidx(g, ib, j1b, kb), vvz1 * vz1w3);
+  atomicAdd(jx + cidx(g, ib, j1b, k1b), vvx1 * vx1w4);
+  atomicAdd(jy + cidx(g, i1b, jb, k1b), vvy1 * vy1w4);
+  atomicAdd(jz + cidx(g, i1b, j1b, kb), vvz1 * vz1w4);
+  atomicAdd(jx + cidx(g, ia, ja, ka), vvx2 * vx2w1);
+  atomicAdd(jy + cidx(g, ia, ja, ka), vvy2 * vy2w1);
+  atomicAdd(jz + cidx(g, ia, ja, ka), vvz2 * vz2w1);
+  atomicAdd(jx + cidx(g, ia, j1a, ka), vvx2 * vx2w2);
+  atomicAdd(jy + cidx(g, i1a, ja, ka), vvy2 * vy2w2);
+  atomicAdd(jz + cidx(g, i1a, ja, ka), vvz2 * vz2w2);
+  atomicAdd(jx + cidx(g, ia, ja, k1a), vvx2 * vx2w3);
+  atomicAdd(jy + cidx(g, ia, ja, k1a), vvy2 * vy2w3);
+  atomicAdd(jz + cidx(g, ia, j1a, ka), vvz2 * vz2w3);
+  atomicAdd(jx + cidx(g, ia, j1a, k1a), vvx2 * vx2w4);
+  atomicAdd(jy + cidx(g, i1a, ja, k1a), vvy2 * vy2w4);
+  atomicAdd(jz + cidx(g, i1a, j1a, ka), vvz2 * vz2w4);
+}
+
+dim3 species_grid(const SpeciesTab &sp) {
+  int64_t mx = 1;
+  for (int s = 0; s < sp.nspec; s++) mx = sp.cnt[s] > mx ? sp.cnt[s] : mx;
+  return dim3((unsigned)((mx + kBlock - 1) / kBlock), (unsigned)sp.nspec, 1);
+}
+
+}  // namespace
+
+void launch_relocate64(emg_ctx *c, int elo[3], int eext[3]) {
+  int64_t nn = (int64_t)(c->g.nx + 1) * (c->g.ny + 1) * (c->g.nz + 1);
+  const emg_config &f = c->cfg;
+  timer_begin(c, kTRelocate);
+  hipLaunchKernelGGL(k_relocate, dim3((unsigned)((nn + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                     c->stream, c->eb_dev, elo[0], elo[1], elo[2], eext[0], eext[1],
+                     c->cfg.precision == EMG_PLAIN_FP64 ? c->node64 : nullptr,
+                     c->cfg.precision == EMG_FP32_FIXED ? c->node32 : nullptr, c->g, f.e0[0],
+                     f.e0[1], f.e0[2], f.b0[0], f.b0[1], f.b0[2]);
+  timer_end(c, kTRelocate);
+}
+
+void launch_positn64(emg_ctx *c, int ustep) {
+  timer_begin(c, kTPush);
+  hipLaunchKernelGGL(k_positn64, species_grid(c->sp), dim3(kBlock), 0, c->stream, c->s64, c->sp,
+                     c->g, ustep, c->dflag);
+  timer_end(c, kTPush);
+}
+
+void launch_push_es64(emg_ctx *c, int ustep) {
+  timer_begin(c, kTPush);
+  hipLaunchKernelGGL(k_push_es64, species_grid(c->sp), dim3(kBlock), 0, c->stream, c->s64, c->sp,
+                     c->node64, c->g, ustep, c->dflag);
+  timer_end(c, kTPush);
+}
+
+void launch_deposit64(emg_ctx *c) {
+  launch_zero(c->rho64, sizeof(double) * c->ncell * c->sp.nspec, c->stream);
+  timer_begin(c, kTDeposit);
+  hipLaunchKernelGGL(k_deposit64, species_grid(c->sp), dim3(kBlock), 0, c->stream, c->s64, c->sp,
+                     c->g, c->rho64, c->ncell, c->dflag);
+  timer_end(c, kTDeposit);
+}
+
+void launch_push_em64(emg_ctx *c, int ustep) {
+  launch_zero(c->j64, sizeof(double) * 3 * c->ncell * c->sp.nspec, c->stream);
+  timer_begin(c, kTPush);
+  hipLaunchKernelGGL(k_push_em64, species_grid(c->sp), dim3(kBlock), 0, c->stream, c->s64, c->sp,
+                     c->node64, c->g, ustep, c->j64, c->ncell, c->dflag);
+  timer_end(c, kTPush);
+}
+
+}  // namespace emg

@@ -1,0 +1,217 @@
+/*
+ * emgpu.h -- C ABI of the MI355X particle hot path of EMSES (libemgpu.so).
+ *
+ * One library context owns one GPU-resident particle population (all species
+ * of one periodic domain that a single GPU owns) and the grid arrays the path
+ * reads and writes.  Every entry point is extern "C", takes plain pointers and
+ * sizes, returns 0 (EMG_OK) or a negative EMG_ERR_* code and never aborts; the
+ * message of the last error is available from emg_last_error().  Calls that
+ * only launch device work (emg_positn, emg_push_es, emg_deposit_rho,
+ * emg_push_em, emg_sort, emg_init_device) return once the work is enqueued on
+ * the context stream; device-side errors they detect (a particle outside the
+ * box, a lost particle) are reported by the next synchronising call
+ * (emg_get_*, emg_energy, emg_download_particles, emg_synchronize).  Host
+ * buffers passed in may be reused as soon as a call returns.  Contexts are
+ * independent; one context must not be used from two host threads at once.
+ *
+ * The reference interfaces each entry point replaces are cited as
+ * path:line in sakatataisei1226/ctca.  The reference passes its state through
+ * Fortran module globals (src/main/allcom.F90:33-84); here it is passed
+ * explicitly.  Grid arrays cross this boundary in the reference's own Fortran
+ * layouts: component index fastest, then i, j, k, with explicit bounds lo..hi.
+ */
+#ifndef EMGPU_H
+#define EMGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EMG_MAX_SPECIES 8
+
+/* return codes */
+#define EMG_OK 0
+#define EMG_ERR_ARG (-1)         /* invalid argument                           */
+#define EMG_ERR_HIP (-2)         /* HIP runtime failure                         */
+#define EMG_ERR_NOMEM (-3)       /* device allocation failed                    */
+#define EMG_ERR_STATE (-4)       /* call out of order (e.g. push before fields) */
+#define EMG_ERR_DOMAIN (-5)      /* particle outside [0, L] or non-finite       */
+#define EMG_ERR_UNSUPPORTED (-6) /* record kind / option not supported          */
+#define EMG_ERR_CAPACITY (-7)    /* more particles than the context capacity    */
+
+/* precision forms of the path */
+#define EMG_PLAIN_FP64 0  /* FP64 state, reference arithmetic, FP64 atomics     */
+#define EMG_FP32_FIXED 1  /* FP32 cell index + in-cell offset, int64 fixed point */
+
+/*
+ * 64-byte particle record, identical to struct S_particle /
+ * type(oh_particle): lib/ohhelp/ohhelp-1.1.1/oh_part.h:23-28,
+ * lib/ohhelp/ohhelp-1.1.1/oh_type.F90:15-28.
+ */
+typedef struct {
+  double x, y, z, vx, vy, vz;
+  int32_t pid, preside, nid, spec;
+} emg_particle;
+
+/*
+ * Context configuration.  All physical values are in the reference's S-units
+ * after rescal (src/main/rescal.F90:38-193): dr = 1, dt = 2, qm(1) = 1.
+ */
+typedef struct {
+  int32_t n[3];        /* cells per axis nx, ny, nz (slx = nx, ...)          */
+  int32_t nspec;       /* species count, <= EMG_MAX_SPECIES                  */
+  int32_t precision;   /* EMG_PLAIN_FP64 or EMG_FP32_FIXED                   */
+  int32_t tile;        /* FP32_FIXED: tile edge in cells (0 = default 4)     */
+  int32_t track_pid;   /* FP32_FIXED: carry pid through every move (+8 B per
+                          particle-step); without it download returns pid 0  */
+  int32_t reserved;    /* must be 0                                          */
+  int64_t capacity[EMG_MAX_SPECIES]; /* max particles per species            */
+  double q[EMG_MAX_SPECIES];  /* charge per macro particle, allcom q(is)     */
+  double qm[EMG_MAX_SPECIES]; /* charge-to-mass ratio, allcom qm(is)         */
+  double rm[EMG_MAX_SPECIES]; /* mass, allcom rm(is) (energy only)           */
+  double e0[3];        /* uniform applied E added by grdcrt (e0x, e0y, e0z)  */
+  double b0[3];        /* uniform applied B added by grdcrt (b0x, b0y, b0z)  */
+} emg_config;
+
+typedef struct emg_ctx emg_ctx;
+
+/* Context lifetime.  Replaces the allocation part of ohinit
+ * (src/wrapper/ohinit.F90:63-211) for the particle path. */
+int emg_create(const emg_config *cfg, emg_ctx **out);
+int emg_destroy(emg_ctx *ctx);
+
+/* Copy the message of the last failing call on this thread into buf. */
+int emg_last_error(char *buf, size_t len);
+
+/* Optional: run on a caller stream (hipStream_t as void*); NULL = own stream. */
+int emg_set_stream(emg_ctx *ctx, void *hip_stream);
+int emg_synchronize(emg_ctx *ctx);
+
+/*
+ * Particle upload / download: the species-contiguous segments of pbuf,
+ * src/pcl2field/psolve2.F90:102-115 (species is occupies counts[is] records in
+ * order).  Upload keeps records with nid >= 0 in order and drops nid < 0,
+ * which is what the single-rank oh3_transbound compaction does
+ * (lib/ohhelp/ohhelp-1.1.1/ohhelp2.c:209-237, 390-470).  Records must be
+ * alive (preside == 0, OH_PCL_ALIVE, oh_type.F90:31-34) and inside [0, L].
+ * Download writes the records back species-contiguously with spec = is+1,
+ * nid = 0, preside = 0 and the uploaded pid.
+ */
+int emg_upload_particles(emg_ctx *ctx, const emg_particle *pbuf, const int64_t *counts,
+                         int32_t nspec);
+int emg_download_particles(emg_ctx *ctx, emg_particle *pbuf, int64_t *counts);
+int emg_particle_counts(emg_ctx *ctx, int64_t *counts);
+
+/*
+ * grdcrt(ps) (src/field/grdcrt.F90:125-230): upload the staggered Yee fields
+ * eb(6, lo(1):hi(1), lo(2):hi(2), lo(3):hi(3)) and relocate them to the nodes
+ * 0..n on the device, adding e0/b0.  Needs lo <= -1 and hi >= n per axis.
+ */
+int emg_set_fields(emg_ctx *ctx, const double *eb, const int32_t lo[3], const int32_t hi[3]);
+
+/* positn(ps, ustep) (src/pcl/positn.F90:117-254): x += v * ustep, periodic wrap. */
+int emg_positn(emg_ctx *ctx, int32_t ustep);
+
+/*
+ * psolve2(ps, ustep) (src/pcl2field/psolve2.F90:4-548): trilinear gather of
+ * the relocated E/B, Buneman-Boris velocity update, x += v * ustep, periodic
+ * wrap.  In EMG_FP32_FIXED the charge deposition at the new positions is done
+ * in the same pass, so the following emg_deposit_rho costs nothing.
+ */
+int emg_push_es(emg_ctx *ctx, int32_t ustep);
+
+/*
+ * charge(ps, func) + oh3_exchange_borders(rho, CRH) + add_boundary_charge
+ * (src/pcl2field/charge.F90:4-412, src/main/esses.F90:530-562): CIC charge
+ * deposition with the periodic fold.  func = 0 or 2 (2 also keeps the
+ * per-species densities that rhodg(is, ...) holds).
+ */
+int emg_deposit_rho(emg_ctx *ctx, int32_t func);
+
+/* Read the folded rho(1, lo:hi)^3 (Fortran layout, periodic values on every
+ * index, ghosts included).  emg_get_rhodg reads species is (0-based). */
+int emg_get_rho(emg_ctx *ctx, double *rho, const int32_t lo[3], const int32_t hi[3]);
+int emg_get_rhodg(emg_ctx *ctx, int32_t is, double *rho, const int32_t lo[3],
+                  const int32_t hi[3]);
+
+/* energy (src/etc/energy.F90:30-48): engk[3*is + d] =
+ * 0.5 rm(is) sum v_d^2 / (slx sly slz), FP64. */
+int emg_energy(emg_ctx *ctx, double *engk);
+
+/*
+ * psolve1(ps, ustep, func) (src/pcl2field/psolve1.F90:4-857): the same push
+ * with floor() cell indices plus the Umeda (2003) zigzag current of each
+ * species, aj += q(is) * J(is), followed by oh3_exchange_borders(aj, CAJ) +
+ * add_boundary_current (psolve1.F90:859-1079) as a periodic fold.  The
+ * current accumulators are cleared by each call (psuper1 zeroes aj,
+ * src/pcl/psuper1.F90:88-91).
+ */
+int emg_push_em(emg_ctx *ctx, int32_t ustep);
+int emg_get_current(emg_ctx *ctx, double *aj, const int32_t lo[3], const int32_t hi[3]);
+
+/*
+ * spsort(ps) (src/etc/spsort.F90:4-39 -> lib/psort/psort-struct.c:7-9):
+ * reorder each species segment by cell (z slowest, then y, then x).  The
+ * reference's comparator (int)(a->z - b->z) is a z-slab order with an
+ * implementation-defined order inside a slab; this sort refines it.  The
+ * order of particles inside one cell is not specified.
+ */
+int emg_sort(emg_ctx *ctx);
+
+/*
+ * Seeded synthetic particles generated on the device (large configurations
+ * whose host generation would dominate): counter-based SplitMix64 stream,
+ * uniform positions (or an exponential density ramp along z), Box-Muller
+ * Maxwellian velocities rotated by t (inipcl.F90:310-320 rotation).  Not the
+ * reference's MT19937 stream; see emg_host_inipcl for that.
+ */
+typedef struct {
+  int64_t count;
+  double vth_perp, vth_para; /* peth, path (S-units)                        */
+  double drift[3];           /* added after rotation                        */
+  double t[9];               /* rotation T(phiz, phixy), row-major          */
+  double ramp_ratio;         /* >1: density ~ exp(z ln(ratio)/Lz); else uniform */
+} emg_device_species;
+
+int emg_init_device(emg_ctx *ctx, const emg_device_species *spec, int32_t nspec,
+                    uint64_t seed);
+
+/*
+ * Host mirror of the reference's seeded initial state: MT19937 seeded with
+ * `seed`, the three res53 draws of inital's RNG self test
+ * (src/main/inital.F90:703-737), then inipcl's draw order per species
+ * (src/pcl/inipcl.F90:77-104, 266-326), in EMSES-U units (before rescal).
+ * Writes sum(npin) records to out (species-contiguous).  Host only.
+ */
+typedef struct {
+  int64_t npin;
+  double peth, path, spe, speth, spa;
+  double vdri, vdthz, vdthxy;
+  double vdx, vdy, vdz;
+} emg_host_species;
+
+int emg_host_inipcl(const emg_host_species *spec, int32_t nspec, const int32_t n[3],
+                    const double t[9], uint32_t seed, emg_particle *out);
+
+/*
+ * Kernel timing with HIP events recorded on the context stream around every
+ * launch of a kind (which: 0 push/move, 1 deposit, 2 sort, 3 relocate,
+ * 4 energy).  No reference counterpart: the reference times its whole loop
+ * with MPI_Wtime (src/main/esses.F90:106-143).  Stats synchronise the stream.
+ */
+#define EMG_KERNEL_PUSH 0
+#define EMG_KERNEL_DEPOSIT 1
+#define EMG_KERNEL_SORT 2
+#define EMG_KERNEL_RELOCATE 3
+#define EMG_KERNEL_ENERGY 4
+int emg_enable_timing(emg_ctx *ctx, int32_t on);
+int emg_kernel_stats(emg_ctx *ctx, int32_t which, double *total_ms, int64_t *launches);
+int emg_reset_kernel_stats(emg_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
