@@ -185,6 +185,13 @@ __device__ __forceinline__ void fx_one(const FxArgs &a, int s, int b_own, bool a
     vz = a.in.vz[src];
   }
   int ix = cell & kCellMask, iy = (cell >> kCellBits) & kCellMask, iz = cell >> (2 * kCellBits);
+  if ((unsigned)ix >= (unsigned)g.nx || (unsigned)iy >= (unsigned)g.ny ||
+      (unsigned)iz >= (unsigned)g.nz) {  // never for a stored particle; keeps indexing in bounds
+    if (act) atomicOr(a.flag, kFlagDomain);
+    ix = 0;
+    iy = 0;
+    iz = 0;
+  }
   if (PUSH && act) {
     float f[6];
     int lx = ix - ox, ly = iy - oy, lz = iz - oz;
@@ -452,6 +459,16 @@ __global__ void k_fx_caps(const int32_t *count_in, int32_t *cap, int32_t *count_
     count_out[b] = 0;
   }
   if (b < nspec) spill_out[b] = 0;
+}
+
+// Arrivals into a full bucket still bump its counter (the particle itself
+// goes to the spill region), so after a step the counters are clamped back to
+// the number of particles actually stored: count <= capacity, spill <= cap.
+__global__ void k_fx_clamp(int32_t *count, const int32_t *cap, int nb, int32_t *spill, int nspec,
+                           int32_t spill_cap) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < nb) count[b] = min(count[b], cap[b]);
+  if (b < nspec) spill[b] = min(spill[b], spill_cap);
 }
 
 // ---- bucketing of input ranges (flat species segments, spill regions or
@@ -850,6 +867,9 @@ void launch_fx_step(emg_ctx *c, int ustep, bool push, bool deposit, bool em) {
     case 8: step_t<8>(c, a, push, deposit, grid); break;
   }
   timer_end(c, kTPush);
+  hipLaunchKernelGGL(k_fx_clamp, dim3((B.nb + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream,
+                     B.count[1 - B.cur], B.capb, B.nb, B.spill[1 - B.cur], c->sp.nspec,
+                     (int32_t)B.spill_cap);
   B.cur = 1 - B.cur;
 }
 

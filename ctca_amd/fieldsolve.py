@@ -30,7 +30,7 @@ def poisson(rho):
     div[0, 0, 0] = 1.0
     spec /= div
     spec[0, 0, 0] = 0.0
-    return np.fft.irfftn(spec, s=rho.shape)
+    return np.fft.irfftn(spec, s=rho.shape, axes=(0, 1, 2))
 
 
 def efield(phi):
