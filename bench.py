@@ -1,0 +1,228 @@
+"""Benchmark of the particle hot path on one MI355X.
+
+Workload (default): the magnetised scale-up -- 128^3 periodic cells, two
+species of 64 particles per cell (268,435,456 particles), B along z with
+wce/wpe = 0.5, vth_e 1.0, vth_i 0.05, mass ratio 100, dt 0.05.  Particles
+are generated on the device from a seeded counter-based stream; the fields
+are E = -grad(phi) of the initial charge density (host Poisson solve) plus
+the uniform B0.
+
+A step is one push + deposit pass over every particle:
+  * FP32 form (the headline `value`): emg_push_es in EMG_FP32_FIXED -- one
+    fused kernel: LDS-staged gather, FP32 Boris, cell+offset move, int64
+    fixed-point CIC deposit, re-bucketing;
+  * FP64 form (beside it): emg_push_es + emg_deposit_rho in EMG_PLAIN_FP64,
+    the reference arithmetic (psolve2 then charge).
+The host field solve is not part of a step (it stays on the host in the
+reference); inputs are resident in HBM before the timed region.
+
+Prints one JSON line.  `roofline` is the fused FP32 kernel: algorithmic bytes
+per launch (56 B per particle + 24 B per field node + 8 B per charge node and
+species) over its average launch time from HIP events on the context stream.
+`cpu_baseline` times the oracle (FP64 C restatement of psolve2 + charge +
+fold, one thread) on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-steps/s push+deposit, FP32/fixed-pt vs FP64 GPU; adjoint/forward time"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_fields(ctx, su, n):
+    """E of the initial density (host Poisson) as the eb halo array."""
+    from ctca_amd import fieldsolve
+    ctx.deposit_rho(0)
+    rho = ctx.rho() + fieldsolve.background(su.q, su.npin, n)
+    phi = fieldsolve.poisson(rho)
+    return fieldsolve.eb_halo(fieldsolve.efield(phi))
+
+
+def time_steps(ctx, form, steps, warmup):
+    from ctca_amd import emg
+
+    def one():
+        ctx.push_es(2)
+        if form == "fp64":
+            ctx.deposit_rho(0)
+
+    for _ in range(warmup):
+        one()
+    ctx.synchronize()
+    ctx.enable_timing(True)
+    ctx.reset_stats()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        one()
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    stats = {k: ctx.kernel_stats(k) for k in (emg.KERNEL_PUSH, emg.KERNEL_DEPOSIT)}
+    ctx.enable_timing(False)
+    return t1 - t0, stats
+
+
+def cpu_baseline(w, su, eb, nsample, steps):
+    """The oracle on host cores: psolve2 + charge + periodic fold, one thread,
+    particles first ordered by the reference's own particle_sort_ (z slabs)."""
+    import ctypes as C
+
+    import oracle
+    n = w.n
+    g = oracle.rng_start(w.seed)
+    parts = []
+    counts = []
+    for s in w.species:
+        sp = oracle.SpeciesInit(npin=nsample // len(w.species), peth=s.vth, path=s.vth)
+        p = oracle.inipcl_species(g, sp, len(parts) + 1, n, su.t)
+        p["vx"] *= su.renv
+        p["vy"] *= su.renv
+        p["vz"] *= su.renv
+        ref = oracle.ref_lib()
+        if ref is not None:
+            ref.particle_sort_.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+            ref.particle_sort_(p.ctypes.data_as(C.c_void_p), C.byref(C.c_int(len(p))))
+        parts.append(p)
+        counts.append(len(p))
+    p = np.concatenate(parts)
+    wrk = oracle.grdcrt(eb, n, su.e0, su.b0)
+    sl = np.array(n, dtype=float)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        oracle.psolve2(p, counts, su.qm, wrk, n, 2, sl)
+        r = oracle.charge(p, counts, su.q, n)
+        oracle.fold_rho(r, n)
+    t1 = time.perf_counter()
+    total = sum(counts)
+    return {"value": total * steps / (t1 - t0), "unit": "particle-steps/s", "cores": 1,
+            "kind": "port",
+            "sample": f"{n[0]}x{n[1]}x{n[2]} grid, {total:,} particles ({len(counts)} species, "
+                      f"z-sorted by the reference's particle_sort_), {steps} steps of psolve2 + "
+                      f"charge + periodic fold, FP64, 1 thread, {t1 - t0:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="mag128_2sp_64ppc")
+    ap.add_argument("--tile", type=int, default=4)
+    ap.add_argument("--fp64-steps", type=int, default=None)
+    ap.add_argument("--cpu-sample", type=int, default=12_582_912)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the particle path runs on one MI355X that owns the "
+              "whole domain; sharding over several GPUs is a design in DESIGN.md, not a "
+              "measured mode.", file=sys.stderr)
+        sys.exit(2)
+
+    from ctca_amd import emg, units
+    w = units.WORKLOADS[args.workload]()
+    su = units.to_sunits(w)
+    n = w.n
+    nsp = len(w.species)
+    counts = su.npin
+    npart = int(counts.sum())
+    ncell = n[0] * n[1] * n[2]
+    spec = units.device_species(w, su)
+    log(f"[bench] {w.name}: {npart:,} particles, grid {n}")
+
+    # ---- FP32 cell + offset, int64 fixed point (headline)
+    ctx = emg.Context(n, su.q, su.qm, su.rm, counts, precision=emg.FP32_FIXED, e0=su.e0,
+                      b0=su.b0, tile=args.tile)
+    ctx.init_device(spec, w.seed)
+    eb = build_fields(ctx, su, n)
+    ctx.set_fields(eb)
+    ctx.push_es(2)
+    rho32 = [ctx.rho(species=k) for k in range(nsp)]
+    el32, st32 = time_steps(ctx, "fp32", args.steps, args.warmup)
+    ek32 = ctx.energy()
+    ctx.close()
+    v32 = npart * args.steps / el32
+    push_ms, push_n = st32[emg.KERNEL_PUSH]
+    kern_ms = push_ms / max(push_n, 1)
+    bytes_launch = 56.0 * npart + 24.0 * ncell + 8.0 * ncell * nsp
+    achieved = bytes_launch / (kern_ms * 1e-3) / 1e9
+    log(f"[bench] fp32: {el32 / args.steps * 1e3:.3f} ms/step, kernel {kern_ms:.3f} ms, "
+        f"{achieved:.0f} GB/s algorithmic")
+
+    # ---- plain FP64 (reference arithmetic) beside it
+    fp64 = None
+    if not args.no_fp64:
+        steps64 = args.fp64_steps or max(args.steps // 4, 10)
+        ctx = emg.Context(n, su.q, su.qm, su.rm, counts, precision=emg.PLAIN_FP64, e0=su.e0,
+                          b0=su.b0)
+        ctx.init_device(spec, w.seed)
+        ctx.sort()   # spsort at initialisation (esses.F90:277-284)
+        ctx.set_fields(eb)
+        ctx.push_es(2)
+        ctx.deposit_rho(2)
+        rho64 = [ctx.rho(species=k) for k in range(nsp)]
+        el64, st64 = time_steps(ctx, "fp64", steps64, args.warmup)
+        ctx.close()
+        v64 = npart * steps64 / el64
+        pm, pn = st64[emg.KERNEL_PUSH]
+        dm, dn = st64[emg.KERNEL_DEPOSIT]
+        derr = [float(np.abs(rho32[k] - rho64[k]).max() / np.abs(rho64[k]).max()) for k in range(nsp)]
+        fp64 = {"particle_steps_per_s": v64, "ms_per_step": el64 / steps64 * 1e3, "steps": steps64,
+                "push_kernel_ms": pm / max(pn, 1), "deposit_kernel_ms": dm / max(dn, 1),
+                "push_gbs_algorithmic": (96.0 * npart + 48.0 * (n[0] + 1) * (n[1] + 1) * (n[2] + 1))
+                / (pm / max(pn, 1) * 1e-3) / 1e9,
+                "deposit_gbs_algorithmic": (24.0 * npart + 8.0 * ncell * nsp) / (dm / max(dn, 1) * 1e-3) / 1e9,
+                "fp32_over_fp64": v32 / v64,
+                "density_err_fp32_vs_fp64_one_step": derr}
+        log(f"[bench] fp64: {el64 / steps64 * 1e3:.3f} ms/step; fp32/fp64 = {v32 / v64:.2f}x; "
+            f"density err {derr}")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(w, su, eb, args.cpu_sample, args.cpu_steps)
+        log(f"[bench] cpu baseline {cpu['value']:.3e} particle-steps/s")
+
+    out = {
+        "metric": METRIC,
+        "value": v32,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el32 / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded Maxwellian particles generated on the device, E from the "
+                "initial density (host Poisson), uniform B0",
+        "config": {"workload": w.name, "grid": list(n), "particles": npart, "species": nsp,
+                   "ppc_per_species": npart // nsp // ncell, "dt": w.dt, "wc": w.wc,
+                   "form": "FP32 cell index + in-cell offset push, int64 fixed-point CIC charge "
+                           f"(2^24 per unit weight), tile {args.tile}^3 cells",
+                   "kinetic_energy_fp32": ek32.tolist(),
+                   "fp64": fp64},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_fx_step<4,true,true> (push + deposit + re-bucket)",
+                     "kernel_ms": kern_ms, "bytes_per_launch": bytes_launch},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
