@@ -329,6 +329,10 @@ int emg_destroy(emg_ctx *c) {
     dfree(B.spill[k]);
   }
   dfree(B.capb);
+  dfree(B.front);
+  dfree(B.back);
+  dfree(B.mv);
+  dfree(B.mv_count);
   dfree(B.hist);
   dfree(B.cursor);
   dfree(c->eb_dev);

@@ -65,7 +65,13 @@ struct FxBuckets {
   int32_t *capb;       // [nb] capacity of the output buckets of the current step
   int32_t *hist;       // [nb] bucketing scratch: histogram
   int32_t *cursor;     // [nb] bucketing scratch: scatter cursors
-  int32_t *spill[2];   // [1] spill counters
+  int32_t *spill[2];   // [nspec] spill counters
+  int32_t *front;      // [nb] particles that stayed in their bucket (this step)
+  int32_t *back;       // [nb] arrivals appended behind them (this step)
+  void *mv;            // mover list (particles that changed bucket)
+  int32_t *mv_count;   // [1]
+  int64_t mv_cap;
+  int64_t trash;       // slot index that takes the stores of non-staying lanes
   StateFx st[2];
   int cur;             // buffer holding the live particles
 };

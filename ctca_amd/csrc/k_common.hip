@@ -197,32 +197,35 @@ __global__ __launch_bounds__(kBlock) void k_read_current(ReadArgs r, const void 
 }
 
 // ---------------------------------------------------------------- FP64 sort
-__device__ __forceinline__ int64_t cell_key(double x, double y, double z, const Grid &g) {
-  int i = (int)x, j = (int)y, k = (int)z;
-  i = i < 0 ? 0 : (i >= g.nx ? g.nx - 1 : i);
-  j = j < 0 ? 0 : (j >= g.ny ? g.ny - 1 : j);
-  k = k < 0 ? 0 : (k >= g.nz ? g.nz - 1 : k);
-  return ((int64_t)k * g.ny + j) * g.nx + i;
+// The reference's comparator (int)(a->z - b->z) (psort-struct.c:15-20) orders
+// a species by z slab and leaves the order inside a slab to qsort.  The same
+// order here: a counting sort on the slab index int(z); inside a slab the
+// order is whatever the atomic cursors give.  For the GPU this is also the
+// right order: a slab's gathers hit a two-plane window of the nodal field
+// (L2 resident) while its charge atomics spread over the whole slab.
+__device__ __forceinline__ int64_t slab_key(double z, const Grid &g) {
+  int k = (int)z;
+  return k < 0 ? 0 : (k >= g.nz ? g.nz - 1 : k);
 }
 
-__global__ __launch_bounds__(kBlock) void k_sort_hist(State64 s, SpeciesTab sp, Grid g, int64_t ncell,
+__global__ __launch_bounds__(kBlock) void k_sort_hist(State64 s, SpeciesTab sp, Grid g, int64_t nkey,
                                                       int32_t *count) {
   int is = blockIdx.y;
   int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (m >= sp.cnt[is]) return;
   m += sp.off[is];
-  atomicAdd(&count[(int64_t)is * ncell + cell_key(s.x[m], s.y[m], s.z[m], g)], 1);
+  atomicAdd(&count[(int64_t)is * nkey + slab_key(s.z[m], g)], 1);
 }
 
 __global__ __launch_bounds__(kBlock) void k_sort_scatter(State64 s, State64 t, SpeciesTab sp, Grid g,
-                                                         int64_t ncell, const int64_t *start,
+                                                         int64_t nkey, const int64_t *start,
                                                          int32_t *cursor) {
   int is = blockIdx.y;
   int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (m >= sp.cnt[is]) return;
   m += sp.off[is];
-  int64_t key = (int64_t)is * ncell + cell_key(s.x[m], s.y[m], s.z[m], g);
-  int64_t d = sp.off[is] + (start[key] - start[(int64_t)is * ncell]) + atomicAdd(&cursor[key], 1);
+  int64_t key = (int64_t)is * nkey + slab_key(s.z[m], g);
+  int64_t d = sp.off[is] + (start[key] - start[(int64_t)is * nkey]) + atomicAdd(&cursor[key], 1);
   t.x[d] = s.x[m];
   t.y[d] = s.y[m];
   t.z[d] = s.z[m];
@@ -361,7 +364,8 @@ void launch_read_current(emg_ctx *c, double *dst, const int lo[3], const int hi[
 }
 
 int sort64(emg_ctx *c) {
-  int64_t len = c->ncell * c->sp.nspec;
+  int64_t nkey = c->g.nz;
+  int64_t len = nkey * c->sp.nspec;
   if (c->sort_len < len) {
     if (c->sort_count) (void)hipFree(c->sort_count);
     if (c->sort_cursor) (void)hipFree(c->sort_cursor);
@@ -395,11 +399,11 @@ int sort64(emg_ctx *c) {
   launch_zero(c->sort_cursor, sizeof(int32_t) * len, c->stream);
   dim3 grid = species_grid(c->sp);
   timer_begin(c, kTSort);
-  hipLaunchKernelGGL(k_sort_hist, grid, dim3(kBlock), 0, c->stream, c->s64, c->sp, c->g, c->ncell,
+  hipLaunchKernelGGL(k_sort_hist, grid, dim3(kBlock), 0, c->stream, c->s64, c->sp, c->g, nkey,
                      c->sort_count);
   exclusive_scan_i32(c->sort_count, c->sort_start, len, c->scan_tmp, c->stream);
   hipLaunchKernelGGL(k_sort_scatter, grid, dim3(kBlock), 0, c->stream, c->s64, c->t64, c->sp, c->g,
-                     c->ncell, c->sort_start, c->sort_cursor);
+                     nkey, c->sort_start, c->sort_cursor);
   timer_end(c, kTSort);
   State64 tmp = c->s64;
   c->s64 = c->t64;

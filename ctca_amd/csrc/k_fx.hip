@@ -53,7 +53,18 @@ struct FxArgs {
   float gstep;
   double *ke_part;  // [(nb + nspec * kSpillBlocksPerSpecies) * 3]
   int *flag;
+  int32_t *front, *back;
+  struct MoverRec *mv;
+  int32_t *mv_count;
+  int64_t mv_cap, trash;
 };
+
+// a particle that left its bucket: written by the push, placed by k_fx_arrive
+struct MoverRec {
+  int dest, cell, pid, pad;
+  float fx, fy, fz, vx, vy, vz;
+};
+constexpr int kOutbox = 256;  // movers staged in LDS per workgroup
 
 __device__ __forceinline__ void gather_global(const float *__restrict__ node32, const Grid &g,
                                               int ix, int iy, int iz, float fx, float fy,
@@ -165,26 +176,38 @@ __device__ __forceinline__ void fixed_weights(float fx, float fy, float fz, int 
   W[7] = (1 << kFixShift) - sum;
 }
 
+struct Pcl {
+  int cell, pid;
+  float fx, fy, fz, vx, vy, vz;
+};
+
+// one particle's state from a structure-of-arrays buffer; `base` is the
+// (uniform) first slot of the range, `i` the 32-bit offset inside it
+__device__ __forceinline__ void load_pcl(const StateFx &in, int64_t base, unsigned i, Pcl &p) {
+  p.cell = (in.cell + base)[i];
+  p.pid = in.pid ? (in.pid + base)[i] : 0;
+  p.fx = (in.fx + base)[i];
+  p.fy = (in.fy + base)[i];
+  p.fz = (in.fz + base)[i];
+  p.vx = (in.vx + base)[i];
+  p.vy = (in.vy + base)[i];
+  p.vz = (in.vz + base)[i];
+}
+
+// The common case of every step is handled in straight-line code (LDS
+// gather, LDS charge tile, staying in the own bucket); the rare cases (a
+// particle outside the staged tile, a move beyond the charge halo, a full
+// destination bucket) sit behind wave-uniform branches taken only when some
+// lane of the wave needs them.
 template <int T, bool PUSH, bool DEPOSIT>
-__device__ __forceinline__ void fx_one(const FxArgs &a, int s, int b_own, bool act, int64_t src,
-                                       int ox, int oy, int oz, bool lds_ok, const float4 *fA,
-                                       const float2 *fB, unsigned long long *rh,
-                                       double ke[3]) {
+__device__ __forceinline__ void fx_one(const FxArgs &a, int s, int b_own, bool act, Pcl p,
+                                       int ox, int oy, int oz, bool lds_ok, int64_t own_start,
+                                       int *n_stay, int *n_out, MoverRec *outbox,
+                                       const float4 *fA, const float2 *fB,
+                                       unsigned long long *rh, double ke[3]) {
   constexpr int N3 = T + 3;
   const Grid &g = a.g;
-  int cell = 0, pid = 0;
-  float fx = 0.f, fy = 0.f, fz = 0.f, vx = 0.f, vy = 0.f, vz = 0.f;
-  if (act) {
-    if (a.in.pid) pid = a.in.pid[src];
-    cell = a.in.cell[src];
-    fx = a.in.fx[src];
-    fy = a.in.fy[src];
-    fz = a.in.fz[src];
-    vx = a.in.vx[src];
-    vy = a.in.vy[src];
-    vz = a.in.vz[src];
-  }
-  int ix = cell & kCellMask, iy = (cell >> kCellBits) & kCellMask, iz = cell >> (2 * kCellBits);
+  int ix = p.cell & kCellMask, iy = (p.cell >> kCellBits) & kCellMask, iz = p.cell >> (2 * kCellBits);
   if ((unsigned)ix >= (unsigned)g.nx || (unsigned)iy >= (unsigned)g.ny ||
       (unsigned)iz >= (unsigned)g.nz) {  // never for a stored particle; keeps indexing in bounds
     if (act) atomicOr(a.flag, kFlagDomain);
@@ -192,84 +215,117 @@ __device__ __forceinline__ void fx_one(const FxArgs &a, int s, int b_own, bool a
     iy = 0;
     iz = 0;
   }
-  if (PUSH && act) {
+  float fx = p.fx, fy = p.fy, fz = p.fz, vx = p.vx, vy = p.vy, vz = p.vz;
+  if (PUSH) {
     float f[6];
     int lx = ix - ox, ly = iy - oy, lz = iz - oz;
-    if (lds_ok && (unsigned)lx < (unsigned)T && (unsigned)ly < (unsigned)T &&
-        (unsigned)lz < (unsigned)T)
-      gather_lds<T>(fA, fB, lx, ly, lz, fx, fy, fz, f);
-    else
-      gather_global(a.node32, g, ix, iy, iz, fx, fy, fz, f);
+    bool in_tile = lds_ok && (unsigned)lx < (unsigned)T && (unsigned)ly < (unsigned)T &&
+                   (unsigned)lz < (unsigned)T;
+    if (!in_tile) {
+      lx = 0;
+      ly = 0;
+      lz = 0;
+    }
+    gather_lds<T>(fA, fB, lx, ly, lz, fx, fy, fz, f);
+    if (__ballot(act && !in_tile)) {
+      if (act && !in_tile) gather_global(a.node32, g, ix, iy, iz, fx, fy, fz, f);
+    }
     boris32(vx, vy, vz, f, a.qm[s] * a.gstep * 0.5f);
-    ke[0] += (double)vx * (double)vx;
-    ke[1] += (double)vy * (double)vy;
-    ke[2] += (double)vz * (double)vz;
+    if (act) {
+      ke[0] += (double)vx * (double)vx;
+      ke[1] += (double)vy * (double)vy;
+      ke[2] += (double)vz * (double)vz;
+    }
   }
-  if (act) {
-    move_axis(fx, ix, vx, a.gstep, g.nx, a.flag);
-    move_axis(fy, iy, vy, a.gstep, g.ny, a.flag);
-    move_axis(fz, iz, vz, a.gstep, g.nz, a.flag);
-  }
-  if (DEPOSIT && act) {
+  move_axis(fx, ix, vx, a.gstep, g.nx, a.flag);
+  move_axis(fy, iy, vy, a.gstep, g.ny, a.flag);
+  move_axis(fz, iz, vz, a.gstep, g.nz, a.flag);
+  if (DEPOSIT) {
     int W[8];
     fixed_weights(fx, fy, fz, W);
     int rx = wrapc(ix - ox + 1, g.nx), ry = wrapc(iy - oy + 1, g.ny), rz = wrapc(iz - oz + 1, g.nz);
-    if (lds_ok && rx <= T + 1 && ry <= T + 1 && rz <= T + 1) {
+    bool in_halo = lds_ok && rx <= T + 1 && ry <= T + 1 && rz <= T + 1;
+    if (act && in_halo) {
+      int q0 = (rz * N3 + ry) * N3 + rx;
 #pragma unroll
       for (int q = 0; q < 8; q++) {
         int a2 = q & 1, b2 = (q >> 1) & 1, c2 = q >> 2;
-        atomicAdd(&rh[((rz + c2) * N3 + (ry + b2)) * N3 + (rx + a2)],
-                  (unsigned long long)(long long)W[q]);
+        atomicAdd(&rh[q0 + (c2 * N3 + b2) * N3 + a2], (unsigned long long)(long long)W[q]);
       }
-    } else {
-      unsigned long long *r = a.rho + (int64_t)s * a.ncell;
-      int xs[2] = {ix, ix + 1 == g.nx ? 0 : ix + 1};
-      int ys[2] = {iy, iy + 1 == g.ny ? 0 : iy + 1};
-      int zs[2] = {iz, iz + 1 == g.nz ? 0 : iz + 1};
+    }
+    if (__ballot(act && !in_halo)) {
+      if (act && !in_halo) {
+        unsigned long long *r = a.rho + (int64_t)s * a.ncell;
+        int xs[2] = {ix, ix + 1 == g.nx ? 0 : ix + 1};
+        int ys[2] = {iy, iy + 1 == g.ny ? 0 : iy + 1};
+        int zs[2] = {iz, iz + 1 == g.nz ? 0 : iz + 1};
 #pragma unroll
-      for (int q = 0; q < 8; q++) {
-        int a2 = q & 1, b2 = (q >> 1) & 1, c2 = q >> 2;
-        atomicAdd(&r[((int64_t)zs[c2] * g.ny + ys[b2]) * g.nx + xs[a2]],
-                  (unsigned long long)(long long)W[q]);
+        for (int q = 0; q < 8; q++) {
+          int a2 = q & 1, b2 = (q >> 1) & 1, c2 = q >> 2;
+          atomicAdd(&r[((int64_t)zs[c2] * g.ny + ys[b2]) * g.nx + xs[a2]],
+                    (unsigned long long)(long long)W[q]);
+        }
       }
     }
   }
-  // ---- re-bucketing: destination bucket of the new cell
+  // ---- re-bucketing: destination bucket of the new cell.  Particles that
+  // stay take consecutive slots at the front of their own output bucket (LDS
+  // counter); the others go to the LDS outbox and k_fx_arrive appends them to
+  // their new bucket after every workgroup has finished.  No global atomic
+  // with a return value sits in this path, and every lane stores (a lane
+  // without a slot stores to the trash slot), so loads of the next particle
+  // stay in flight across the whole step.
   int dest = s * a.ntiles + ((iz / T) * a.nty + (iy / T)) * a.ntx + (ix / T);
   bool stay = act && dest == b_own;
+  int lane = threadIdx.x & 63;
+  uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  int64_t dst = a.trash;
   uint64_t m = __ballot(stay);
-  int slot = 0;
   if (m) {
-    int lane = threadIdx.x & 63;
     int leader = __ffsll((unsigned long long)m) - 1;
     int base = 0;
-    if (lane == leader) base = atomicAdd(&a.count_out[b_own], __popcll(m));
+    if (lane == leader) base = atomicAdd(n_stay, __popcll(m));
     base = __shfl(base, leader);
-    uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    slot = base + __popcll(m & lt);
+    if (stay) dst = own_start + base + __popcll(m & lt);
   }
-  if (act) {
-    if (!stay) slot = atomicAdd(&a.count_out[dest], 1);
-    int64_t dst;
-    if (slot < a.cap_out[dest]) {
-      dst = a.start_out[dest] + slot;
-    } else {
-      int k = atomicAdd(&a.spill_out[s], 1);
-      if (k >= a.spill_cap) {
-        atomicOr(a.flag, kFlagSpillLost);
-        return;
+  uint64_t mv = __ballot(act && !stay);
+  if (mv) {
+    int leader = __ffsll((unsigned long long)mv) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(n_out, __popcll(mv));
+    base = __shfl(base, leader);
+    if (act && !stay) {
+      int k = base + __popcll(mv & lt);
+      MoverRec r;
+      r.dest = dest;
+      r.cell = pack_cell(ix, iy, iz);
+      r.pid = p.pid;
+      r.pad = 0;
+      r.fx = fx;
+      r.fy = fy;
+      r.fz = fz;
+      r.vx = vx;
+      r.vy = vy;
+      r.vz = vz;
+      if (k < kOutbox) {
+        outbox[k] = r;
+      } else {  // outbox full: straight to the global mover list
+        int g2 = atomicAdd(a.mv_count, 1);
+        if (g2 < a.mv_cap)
+          a.mv[g2] = r;
+        else
+          atomicOr(a.flag, kFlagSpillLost);
       }
-      dst = a.area + (int64_t)s * a.spill_cap + k;
     }
-    a.out.cell[dst] = pack_cell(ix, iy, iz);
-    if (a.out.pid) a.out.pid[dst] = pid;
-    a.out.fx[dst] = fx;
-    a.out.fy[dst] = fy;
-    a.out.fz[dst] = fz;
-    a.out.vx[dst] = vx;
-    a.out.vy[dst] = vy;
-    a.out.vz[dst] = vz;
   }
+  a.out.cell[dst] = pack_cell(ix, iy, iz);
+  if (a.out.pid) a.out.pid[dst] = p.pid;
+  a.out.fx[dst] = fx;
+  a.out.fy[dst] = fy;
+  a.out.fz[dst] = fz;
+  a.out.vx[dst] = vx;
+  a.out.vy[dst] = vy;
+  a.out.vz[dst] = vz;
 }
 
 __device__ __forceinline__ void block_reduce3(double ke[3], double *out) {
@@ -304,14 +360,25 @@ __global__ __launch_bounds__(kBlock) void k_fx_step(FxArgs a) {
   __shared__ float4 fA[PUSH ? N1 * N1 * N1 : 1];
   __shared__ float2 fB[PUSH ? N1 * N1 * N1 : 1];
   __shared__ unsigned long long rh[DEPOSIT ? N3 * N3 * N3 : 1];
+  __shared__ MoverRec outbox[kOutbox];
+  __shared__ int n_stay, n_out;
+  __shared__ int64_t out_base;
   double ke[3] = {0.0, 0.0, 0.0};
   const int bid = blockIdx.x;
   int part = bid;  // energy partial slot: bucket index, then spill blocks
+  int s = 0, b = -1, ox = 0, oy = 0, oz = 0;
+  if (threadIdx.x == 0) {
+    n_stay = 0;
+    n_out = 0;
+  }
   if (bid < a.nb) {
-    const int b = bucket_of_block(bid, a.nb);
+    b = bucket_of_block(bid, a.nb);
     part = b;
-    const int s = b / a.ntiles, t = b - s * a.ntiles;
-    const int ox = (t % a.ntx) * T, oy = ((t / a.ntx) % a.nty) * T, oz = (t / (a.ntx * a.nty)) * T;
+    s = b / a.ntiles;
+    const int t = b - s * a.ntiles;
+    ox = (t % a.ntx) * T;
+    oy = ((t / a.ntx) % a.nty) * T;
+    oz = (t / (a.ntx * a.nty)) * T;
     const int cnt = a.count_in[b];
     const int64_t beg = a.start_in[b];
     if (cnt > 0) {
@@ -320,22 +387,30 @@ __global__ __launch_bounds__(kBlock) void k_fx_step(FxArgs a) {
           int nq = q >> 1;
           int lx = nq % N1, ly = (nq / N1) % N1, lz = nq / (N1 * N1);
           int gx = wrapc(ox + lx, a.g.nx), gy = wrapc(oy + ly, a.g.ny), gz = wrapc(oz + lz, a.g.nz);
-          const float4 *p = reinterpret_cast<const float4 *>(
+          const float4 *pp = reinterpret_cast<const float4 *>(
               a.node32 + 8 * (((int64_t)gz * a.g.ny + gy) * a.g.nx + gx));
           if (q & 1) {
-            float4 B = p[1];
+            float4 B = pp[1];
             fB[nq] = make_float2(B.x, B.y);
           } else {
-            fA[nq] = p[0];
+            fA[nq] = pp[0];
           }
         }
       }
       if (DEPOSIT)
         for (int q = threadIdx.x; q < N3 * N3 * N3; q += kBlock) rh[q] = 0ull;
       __syncthreads();
+      const int64_t own_start = a.start_out[b];
+      // software pipelined: the next particle's state is in flight while the
+      // current one is pushed (the load index is clamped, so every lane loads)
+      Pcl cur, nxt;
+      load_pcl(a.in, beg, min((int)threadIdx.x, cnt - 1), cur);
       for (int base = 0; base < cnt; base += kBlock) {
-        int i = base + threadIdx.x;
-        fx_one<T, PUSH, DEPOSIT>(a, s, b, i < cnt, beg + i, ox, oy, oz, true, fA, fB, rh, ke);
+        int inext = base + kBlock + (int)threadIdx.x;
+        load_pcl(a.in, beg, (unsigned)min(inext, cnt - 1), nxt);
+        fx_one<T, PUSH, DEPOSIT>(a, s, b, base + (int)threadIdx.x < cnt, cur, ox, oy, oz, true,
+                                 own_start, &n_stay, &n_out, outbox, fA, fB, rh, ke);
+        cur = nxt;
       }
       if (DEPOSIT) {
         __syncthreads();
@@ -355,17 +430,69 @@ __global__ __launch_bounds__(kBlock) void k_fx_step(FxArgs a) {
   } else {
     // spill particles of the previous step: global-memory gather and deposit
     int k = bid - a.nb;
-    int s = k / kSpillBlocksPerSpecies, sub = k % kSpillBlocksPerSpecies;
+    s = k / kSpillBlocksPerSpecies;
+    int sub = k % kSpillBlocksPerSpecies;
     int cnt = a.spill_in[s];
     if (cnt > a.spill_cap) cnt = (int)a.spill_cap;
     int64_t beg = a.area + (int64_t)s * a.spill_cap;
     int rounds = (cnt + kBlock * kSpillBlocksPerSpecies - 1) / (kBlock * kSpillBlocksPerSpecies);
+    __syncthreads();
     for (int r = 0; r < rounds; r++) {
       int i = (r * kSpillBlocksPerSpecies + sub) * kBlock + threadIdx.x;
-      fx_one<T, PUSH, DEPOSIT>(a, s, -1, i < cnt, beg + i, 0, 0, 0, false, fA, fB, rh, ke);
+      Pcl cur{};
+      if (i < cnt) load_pcl(a.in, beg, (unsigned)i, cur);
+      fx_one<T, PUSH, DEPOSIT>(a, s, -1, i < cnt, cur, 0, 0, 0, false, 0, &n_stay, &n_out, outbox,
+                               fA, fB, rh, ke);
     }
   }
+  // hand the outbox to the global mover list and publish the stayer count
+  __syncthreads();
+  int nout = min(n_out, kOutbox);
+  if (threadIdx.x == 0) {
+    if (b >= 0) a.front[b] = n_stay;
+    out_base = nout > 0 ? (int64_t)atomicAdd(a.mv_count, nout) : 0;
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < nout; q += kBlock) {
+    int64_t g2 = out_base + q;
+    if (g2 < a.mv_cap)
+      a.mv[g2] = outbox[q];
+    else
+      atomicOr(a.flag, kFlagSpillLost);
+  }
   if (PUSH) block_reduce3(ke, a.ke_part + 3 * (int64_t)part);
+}
+
+// movers of the step: appended behind the stayers of their destination bucket
+// (or, when that is full, to the species' spill region)
+__global__ __launch_bounds__(kBlock) void k_fx_arrive(FxArgs a) {
+  int n = *a.mv_count;
+  if (n > a.mv_cap) n = (int)a.mv_cap;
+  for (int k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) {
+    MoverRec r = a.mv[k];
+    int dest = r.dest;
+    int slot = a.front[dest] + atomicAdd(&a.back[dest], 1);
+    int64_t dst;
+    if (slot < a.cap_out[dest]) {
+      dst = a.start_out[dest] + slot;
+    } else {
+      int s = dest / a.ntiles;
+      int q = atomicAdd(&a.spill_out[s], 1);
+      if (q >= a.spill_cap) {
+        atomicOr(a.flag, kFlagSpillLost);
+        continue;
+      }
+      dst = a.area + (int64_t)s * a.spill_cap + q;
+    }
+    a.out.cell[dst] = r.cell;
+    if (a.out.pid) a.out.pid[dst] = r.pid;
+    a.out.fx[dst] = r.fx;
+    a.out.fy[dst] = r.fy;
+    a.out.fz[dst] = r.fz;
+    a.out.vx[dst] = r.vx;
+    a.out.vy[dst] = r.vy;
+    a.out.vz[dst] = r.vz;
+  }
 }
 
 // standalone charge deposition of the current positions (no move, no bucket change)
@@ -461,13 +588,21 @@ __global__ void k_fx_caps(const int32_t *count_in, int32_t *cap, int32_t *count_
   if (b < nspec) spill_out[b] = 0;
 }
 
+// before a step: arrival counters and the mover list are empty
+__global__ void k_fx_step_prep(int32_t *back, int32_t *mv_count, int nb) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < nb) back[b] = 0;
+  if (b == 0) *mv_count = 0;
+}
+
 // Arrivals into a full bucket still bump its counter (the particle itself
 // goes to the spill region), so after a step the counters are clamped back to
 // the number of particles actually stored: count <= capacity, spill <= cap.
-__global__ void k_fx_clamp(int32_t *count, const int32_t *cap, int nb, int32_t *spill, int nspec,
+__global__ void k_fx_clamp(int32_t *count, const int32_t *front, const int32_t *back,
+                           const int32_t *cap, int nb, int32_t *spill, int nspec,
                            int32_t spill_cap) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b < nb) count[b] = min(count[b], cap[b]);
+  if (b < nb) count[b] = min(front[b] + back[b], cap[b]);
   if (b < nspec) spill[b] = min(spill[b], spill_cap);
 }
 
@@ -683,6 +818,12 @@ FxArgs make_args(emg_ctx *c) {
   a.gstep = 0.f;
   a.ke_part = c->ke_part;
   a.flag = c->dflag;
+  a.front = B.front;
+  a.back = B.back;
+  a.mv = (MoverRec *)B.mv;
+  a.mv_count = B.mv_count;
+  a.mv_cap = B.mv_cap;
+  a.trash = B.trash;
   return a;
 }
 
@@ -757,7 +898,9 @@ int fx_alloc(emg_ctx *c, int64_t ntotal) {
   for (int s = 0; s < c->sp.nspec; s++) maxs = c->cfg.capacity[s] > maxs ? c->cfg.capacity[s] : maxs;
   B.spill_cap = maxs / 64 + 4096;
   if (B.spill_cap > (1 << 30)) B.spill_cap = 1 << 30;
-  B.cap_total = B.area + B.spill_cap * c->sp.nspec;
+  B.cap_total = B.area + B.spill_cap * c->sp.nspec + 1;  // + the trash slot
+  B.trash = B.cap_total - 1;
+  B.mv_cap = ntotal / 4 + 65536;
   for (int k = 0; k < 2; k++) {
     StateFx &S = B.st[k];
     void *p[7];
@@ -783,6 +926,11 @@ int fx_alloc(emg_ctx *c, int64_t ntotal) {
     launch_zero(B.spill[k], sizeof(int32_t) * EMG_MAX_SPECIES, c->stream);
     launch_zero(B.start[k], sizeof(int64_t) * (B.nb + 1), c->stream);
   }
+  if (hipMalloc(&B.front, sizeof(int32_t) * B.nb) != hipSuccess ||
+      hipMalloc(&B.back, sizeof(int32_t) * B.nb) != hipSuccess ||
+      hipMalloc(&B.mv, sizeof(MoverRec) * B.mv_cap) != hipSuccess ||
+      hipMalloc(&B.mv_count, sizeof(int32_t)) != hipSuccess)
+    return set_error(EMG_ERR_NOMEM, "mover list");
   if (hipMalloc(&B.capb, sizeof(int32_t) * B.nb) != hipSuccess ||
       hipMalloc(&B.hist, sizeof(int32_t) * B.nb) != hipSuccess ||
       hipMalloc(&B.cursor, sizeof(int32_t) * B.nb) != hipSuccess)
@@ -853,9 +1001,10 @@ int fx_to64(emg_ctx *c, State64 &dst) {
 void launch_fx_step(emg_ctx *c, int ustep, bool push, bool deposit, bool em) {
   (void)em;
   FxBuckets &B = c->fx;
-  hipLaunchKernelGGL(k_fx_caps, dim3((B.nb + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream,
-                     B.count[B.cur], B.capb, B.count[1 - B.cur], B.spill[1 - B.cur], B.nb,
-                     c->sp.nspec);
+  unsigned gb = (unsigned)((B.nb + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(k_fx_caps, dim3(gb), dim3(kBlock), 0, c->stream, B.count[B.cur], B.capb,
+                     B.count[1 - B.cur], B.spill[1 - B.cur], B.nb, c->sp.nspec);
+  hipLaunchKernelGGL(k_fx_step_prep, dim3(gb), dim3(kBlock), 0, c->stream, B.back, B.mv_count, B.nb);
   exclusive_scan_i32(B.capb, B.start[1 - B.cur], B.nb, c->scan_tmp, c->stream);
   if (deposit) launch_zero(c->rhofx, sizeof(unsigned long long) * c->ncell * c->sp.nspec, c->stream);
   FxArgs a = make_args(c);
@@ -867,9 +1016,9 @@ void launch_fx_step(emg_ctx *c, int ustep, bool push, bool deposit, bool em) {
     case 8: step_t<8>(c, a, push, deposit, grid); break;
   }
   timer_end(c, kTPush);
-  hipLaunchKernelGGL(k_fx_clamp, dim3((B.nb + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream,
-                     B.count[1 - B.cur], B.capb, B.nb, B.spill[1 - B.cur], c->sp.nspec,
-                     (int32_t)B.spill_cap);
+  hipLaunchKernelGGL(k_fx_arrive, dim3(512), dim3(kBlock), 0, c->stream, a);
+  hipLaunchKernelGGL(k_fx_clamp, dim3(gb), dim3(kBlock), 0, c->stream, B.count[1 - B.cur], B.front,
+                     B.back, B.capb, B.nb, B.spill[1 - B.cur], c->sp.nspec, (int32_t)B.spill_cap);
   B.cur = 1 - B.cur;
 }
 
