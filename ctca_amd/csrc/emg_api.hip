@@ -327,12 +327,11 @@ int emg_destroy(emg_ctx *c) {
     dfree(B.start[k]);
     dfree(B.count[k]);
     dfree(B.spill[k]);
+    dfree(B.narr[k]);
+    dfree(B.cap[k]);
   }
-  dfree(B.capb);
   dfree(B.front);
   dfree(B.back);
-  dfree(B.mv);
-  dfree(B.mv_count);
   dfree(B.hist);
   dfree(B.cursor);
   dfree(c->eb_dev);

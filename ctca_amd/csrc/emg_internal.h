@@ -60,18 +60,16 @@ struct FxBuckets {
   int64_t cap_total;   // slots per buffer (bucket area + spill)
   int64_t area;        // bucket area size (spill region starts here)
   int64_t spill_cap;
-  int64_t *start[2];   // [nb + 1]
-  int32_t *count[2];   // [nb]
-  int32_t *capb;       // [nb] capacity of the output buckets of the current step
+  int64_t *start[2];   // [nb + 1] first slot of each bucket
+  int32_t *count[2];   // [nb] particles in each bucket
+  int32_t *narr[2];    // [nb] of which arrivals (front run); the rest are stayers (back run)
+  int32_t *cap[2];     // [nb] slots owned by each bucket
   int32_t *hist;       // [nb] bucketing scratch: histogram
   int32_t *cursor;     // [nb] bucketing scratch: scatter cursors
   int32_t *spill[2];   // [nspec] spill counters
-  int32_t *front;      // [nb] particles that stayed in their bucket (this step)
-  int32_t *back;       // [nb] arrivals appended behind them (this step)
-  void *mv;            // mover list (particles that changed bucket)
-  int32_t *mv_count;   // [1]
-  int64_t mv_cap;
-  int64_t trash;       // slot index that takes the stores of non-staying lanes
+  int32_t *front;      // [nb] stayers written by the bucket's workgroup (this step)
+  int32_t *back;       // [nb] arrivals reserved by other workgroups (this step)
+  int64_t trash;       // slot index that takes the stores of lanes without a slot
   StateFx st[2];
   int cur;             // buffer holding the live particles
 };

@@ -1,32 +1,42 @@
 // k_fx.hip -- the FP32 cell-index + offset form of the particle path with
 // deterministic int64 fixed-point charge deposition.
 //
-// Layout.  Particles of species s live in tile buckets b = s * ntiles + t,
-// where a tile is a T^3 block of cells.  Each bucket occupies the slots
-// [start[b], start[b] + count[b]) of a structure-of-arrays buffer
-// (cell: packed ix | iy << 10 | iz << 20, offsets fx, fy, fz in [0, 1),
-// velocities vx, vy, vz).  Two buffers alternate: a step reads buffer cur and
-// writes every particle into the bucket of its new tile in buffer 1 - cur, so
-// the tile order is kept exactly every step at no extra traffic (each
-// particle is read once and written once).  Output buckets get
-// count + count/4 + 64 slots; a particle that finds its destination full goes
-// to a per-species spill region that the next step processes through the
-// global-memory path, so nothing is lost.
+// Layout.  Particles of species s live in tile buckets b = s * ntiles + t, a
+// tile being a T^3 block of cells.  Structure of arrays per buffer: packed
+// cell ix | iy << 10 | iz << 20, offsets fx, fy, fz in [0, 1), velocities.
+// Bucket b of a buffer owns the slots [start[b], start[b] + cap[b]) and holds
+// count[b] particles in two runs: narr[b] particles that arrived from other
+// buckets at the front, count[b] - narr[b] particles that stayed at the back:
+//   slot(i) = start + i                      for i <  narr
+//   slot(i) = start + cap - count + i        for i >= narr
 //
-// Workgroup = one bucket.  The T+1 nodes per axis of the tile's relocated
-// fields are staged once into LDS; every particle of the bucket gathers from
-// LDS.  Charge goes to an LDS tile of T+3 nodes per axis (one node of halo
-// each side, covering moves of up to one cell) as integer CIC weights scaled
-// by 2^24, whose eight weights are made to sum to exactly 2^24 (the last one
-// takes the remainder), then one global int64 atomic per non-zero LDS node.
-// Integer sums are associative, so rho is bitwise independent of particle
-// order, workgroup scheduling and launch shape.
+// A step reads buffer cur and writes buffer 1 - cur (cap = count + count/4 +
+// 64 per bucket, start = exclusive scan of cap):
+//   * one workgroup per bucket stages the T+1 nodes per axis of the bucket's
+//     relocated fields into LDS, pushes every particle (FP32 Boris), moves it
+//     (cell + offset), deposits its charge into an LDS tile of T+3 nodes per
+//     axis as integer CIC weights (2^24 per unit weight, the eight weights of
+//     a particle summing to exactly 2^24), and writes it back:
+//       - a particle that stays in the tile takes the next slot from the back
+//         of its own output bucket (LDS counter, no global atomic);
+//       - a particle that changes tile goes to an LDS outbox; at the end of
+//         the workgroup the outbox is grouped by destination (the 26
+//         neighbouring tiles), one atomic per destination reserves a run at
+//         the front of that bucket, and the run is written contiguously;
+//   * the charge tile is added to the species' global int64 accumulator with
+//     one atomic per non-zero node.  Integer sums are associative, so rho is
+//     bitwise independent of particle order, scheduling and launch shape.
+// Arrivals beyond a bucket's room (cap - previous count) and particles that
+// moved further than a neighbouring tile go through a per-species spill
+// region that the next step processes through the global-memory path.
 #include "emg_internal.h"
 
 namespace emg {
 namespace {
 
 constexpr int kSpillBlocksPerSpecies = 8;
+constexpr int kOutbox = 256;   // movers staged in LDS per workgroup
+constexpr int kFar = 27;       // outbox code of a destination that is not a neighbour
 
 __device__ __forceinline__ int wrapc(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
@@ -38,14 +48,16 @@ struct FxArgs {
   Grid g;
   int ntx, nty, ntz, ntiles, nb, nspec;
   StateFx in, out;
+  // input buffer
   const int64_t *start_in;
-  const int32_t *count_in;
-  const int32_t *spill_in;
+  const int32_t *count_in, *narr_in, *cap_in, *spill_in;
+  // output buffer
   const int64_t *start_out;
-  int32_t *count_out;
   const int32_t *cap_out;
+  int32_t *front;  // stayers per bucket (written by the bucket's workgroup)
+  int32_t *back;   // arrivals per bucket (atomic)
   int32_t *spill_out;
-  int64_t area, spill_cap;
+  int64_t area, spill_cap, trash;
   const float *node32;
   unsigned long long *rho;
   int64_t ncell;
@@ -53,72 +65,91 @@ struct FxArgs {
   float gstep;
   double *ke_part;  // [(nb + nspec * kSpillBlocksPerSpecies) * 3]
   int *flag;
-  int32_t *front, *back;
-  struct MoverRec *mv;
-  int32_t *mv_count;
-  int64_t mv_cap, trash;
 };
 
-// a particle that left its bucket: written by the push, placed by k_fx_arrive
-struct MoverRec {
-  int dest, cell, pid, pad;
+struct Pcl {
+  int cell, pid;
   float fx, fy, fz, vx, vy, vz;
 };
-constexpr int kOutbox = 256;  // movers staged in LDS per workgroup
 
+struct OutRec {
+  int dest, cell, pid, code, rank;
+  float fx, fy, fz, vx, vy, vz;
+};
+
+__device__ __forceinline__ void load_pcl(const StateFx &in, int64_t slot, Pcl &p) {
+  p.cell = in.cell[slot];
+  p.pid = in.pid ? in.pid[slot] : 0;
+  p.fx = in.fx[slot];
+  p.fy = in.fy[slot];
+  p.fz = in.fz[slot];
+  p.vx = in.vx[slot];
+  p.vy = in.vy[slot];
+  p.vz = in.vz[slot];
+}
+
+__device__ __forceinline__ void store_pcl(const StateFx &out, int64_t dst, int cell, int pid,
+                                          float fx, float fy, float fz, float vx, float vy,
+                                          float vz) {
+  out.cell[dst] = cell;
+  if (out.pid) out.pid[dst] = pid;
+  out.fx[dst] = fx;
+  out.fy[dst] = fy;
+  out.fz[dst] = fz;
+  out.vx[dst] = vx;
+  out.vy[dst] = vy;
+  out.vz[dst] = vz;
+}
+
+// rare path (a particle outside its staged tile): one corner at a time
 __device__ __forceinline__ void gather_global(const float *__restrict__ node32, const Grid &g,
                                               int ix, int iy, int iz, float fx, float fy,
                                               float fz, float f[6]) {
   int ix1 = ix + 1 == g.nx ? 0 : ix + 1;
   int iy1 = iy + 1 == g.ny ? 0 : iy + 1;
   int iz1 = iz + 1 == g.nz ? 0 : iz + 1;
-  float wx[2] = {1.f - fx, fx}, wy[2] = {1.f - fy, fy}, wz[2] = {1.f - fz, fz};
-  int xs[2] = {ix, ix1}, ys[2] = {iy, iy1}, zs[2] = {iz, iz1};
-#pragma unroll
   for (int c = 0; c < 6; c++) f[c] = 0.f;
-#pragma unroll
-  for (int c2 = 0; c2 < 2; c2++)
-#pragma unroll
-    for (int b2 = 0; b2 < 2; b2++)
-#pragma unroll
-      for (int a2 = 0; a2 < 2; a2++) {
-        float w = wx[a2] * wy[b2] * wz[c2];
-        const float4 *p = reinterpret_cast<const float4 *>(
-            node32 + 8 * (((int64_t)zs[c2] * g.ny + ys[b2]) * g.nx + xs[a2]));
-        float4 A = p[0], B = p[1];
-        f[0] += w * A.x;
-        f[1] += w * A.y;
-        f[2] += w * A.z;
-        f[3] += w * A.w;
-        f[4] += w * B.x;
-        f[5] += w * B.y;
-      }
+#pragma unroll 1
+  for (int q = 0; q < 8; q++) {
+    int a2 = q & 1, b2 = (q >> 1) & 1, c2 = q >> 2;
+    float w = (a2 ? fx : 1.f - fx) * (b2 ? fy : 1.f - fy) * (c2 ? fz : 1.f - fz);
+    const float4 *p = reinterpret_cast<const float4 *>(
+        node32 + 8 * (((int64_t)(c2 ? iz1 : iz) * g.ny + (b2 ? iy1 : iy)) * g.nx + (a2 ? ix1 : ix)));
+    float4 A = p[0], B = p[1];
+    f[0] += w * A.x;
+    f[1] += w * A.y;
+    f[2] += w * A.z;
+    f[3] += w * A.w;
+    f[4] += w * B.x;
+    f[5] += w * B.y;
+  }
 }
 
+// trilinear gather from the LDS field tile, two corners (one x pair) at a
+// time: bounds the registers held by LDS reads
 template <int T>
 __device__ __forceinline__ void gather_lds(const float4 *fA, const float2 *fB, int lx, int ly,
                                            int lz, float fx, float fy, float fz, float f[6]) {
   constexpr int N1 = T + 1;
-  float wx[2] = {1.f - fx, fx}, wy[2] = {1.f - fy, fy}, wz[2] = {1.f - fz, fz};
+  float wx0 = 1.f - fx;
+  float wyz[4] = {(1.f - fy) * (1.f - fz), fy * (1.f - fz), (1.f - fy) * fz, fy * fz};
+  int q0 = (lz * N1 + ly) * N1 + lx;
 #pragma unroll
   for (int c = 0; c < 6; c++) f[c] = 0.f;
 #pragma unroll
-  for (int c2 = 0; c2 < 2; c2++)
-#pragma unroll
-    for (int b2 = 0; b2 < 2; b2++)
-#pragma unroll
-      for (int a2 = 0; a2 < 2; a2++) {
-        float w = wx[a2] * wy[b2] * wz[c2];
-        int q = ((lz + c2) * N1 + (ly + b2)) * N1 + (lx + a2);
-        float4 A = fA[q];
-        float2 B = fB[q];
-        f[0] += w * A.x;
-        f[1] += w * A.y;
-        f[2] += w * A.z;
-        f[3] += w * A.w;
-        f[4] += w * B.x;
-        f[5] += w * B.y;
-      }
+  for (int r = 0; r < 4; r++) {
+    int q = q0 + ((r >> 1) * N1 + (r & 1)) * N1;
+    float4 A0 = fA[q], A1 = fA[q + 1];
+    float2 B0 = fB[q], B1 = fB[q + 1];
+    float w0 = wx0 * wyz[r], w1 = fx * wyz[r];
+    f[0] += w0 * A0.x + w1 * A1.x;
+    f[1] += w0 * A0.y + w1 * A1.y;
+    f[2] += w0 * A0.z + w1 * A1.z;
+    f[3] += w0 * A0.w + w1 * A1.w;
+    f[4] += w0 * B0.x + w1 * B1.x;
+    f[5] += w0 * B0.y + w1 * B1.y;
+    __builtin_amdgcn_sched_barrier(0);
+  }
 }
 
 // FP32 Buneman-Boris (same scheme as psolve2.F90:233-257)
@@ -142,7 +173,8 @@ __device__ __forceinline__ void boris32(float &vx, float &vy, float &vz, const f
 }
 
 // move one axis: offset += v * dt, carry whole cells into the index, periodic
-__device__ __forceinline__ void move_axis(float &f, int &i, float v, float dt, int n, int *flag) {
+__device__ __forceinline__ void move_axis(float &f, int &i, float v, float dt, int n, int *flag,
+                                          bool act) {
   float nf = f + v * dt;
   float fl = floorf(nf);
   int d = (int)fl;
@@ -155,7 +187,7 @@ __device__ __forceinline__ void move_axis(float &f, int &i, float v, float dt, i
   if (ni < 0) ni += n;
   else if (ni >= n) ni -= n;
   if ((unsigned)ni >= (unsigned)n) {  // moved more than one box length, or NaN
-    atomicOr(flag, kFlagFar);
+    if (act) atomicOr(flag, kFlagFar);
     ni = ni < 0 ? 0 : n - 1;
   }
   if (!(nf >= 0.f && nf < 1.f)) nf = 0.f;
@@ -165,46 +197,70 @@ __device__ __forceinline__ void move_axis(float &f, int &i, float v, float dt, i
 
 // integer CIC weights summing to exactly 2^24
 __device__ __forceinline__ void fixed_weights(float fx, float fy, float fz, int W[8]) {
-  float wx[2] = {1.f - fx, fx}, wy[2] = {1.f - fy, fy}, wz[2] = {1.f - fz, fz};
+  float wx[2] = {1.f - fx, fx};
+  float wyz[4] = {(1.f - fy) * (1.f - fz) * 16777216.f, fy * (1.f - fz) * 16777216.f,
+                  (1.f - fy) * fz * 16777216.f, fy * fz * 16777216.f};
   int sum = 0;
 #pragma unroll
   for (int q = 0; q < 7; q++) {
-    int a2 = q & 1, b2 = (q >> 1) & 1, c2 = q >> 2;
-    W[q] = (int)rintf(wx[a2] * wy[b2] * wz[c2] * 16777216.f);
+    W[q] = (int)rintf(wx[q & 1] * wyz[q >> 1]);
     sum += W[q];
   }
   W[7] = (1 << kFixShift) - sum;
 }
 
-struct Pcl {
-  int cell, pid;
-  float fx, fy, fz, vx, vy, vz;
-};
-
-// one particle's state from a structure-of-arrays buffer; `base` is the
-// (uniform) first slot of the range, `i` the 32-bit offset inside it
-__device__ __forceinline__ void load_pcl(const StateFx &in, int64_t base, unsigned i, Pcl &p) {
-  p.cell = (in.cell + base)[i];
-  p.pid = in.pid ? (in.pid + base)[i] : 0;
-  p.fx = (in.fx + base)[i];
-  p.fy = (in.fy + base)[i];
-  p.fz = (in.fz + base)[i];
-  p.vx = (in.vx + base)[i];
-  p.vy = (in.vy + base)[i];
-  p.vz = (in.vz + base)[i];
+// destination bucket -> outbox code (26 neighbouring tiles) or kFar
+__device__ __forceinline__ int dest_code(int dest, int b_own, int ntx, int nty, int ntz,
+                                         int ntiles) {
+  if (b_own < 0 || ntx < 3 || nty < 3 || ntz < 3) return kFar;
+  if (dest / ntiles != b_own / ntiles) return kFar;
+  int t = dest % ntiles, o = b_own % ntiles;
+  int dx = t % ntx - o % ntx, dy = (t / ntx) % nty - (o / ntx) % nty,
+      dz = t / (ntx * nty) - o / (ntx * nty);
+  if (dx == ntx - 1) dx = -1;
+  if (dx == 1 - ntx) dx = 1;
+  if (dy == nty - 1) dy = -1;
+  if (dy == 1 - nty) dy = 1;
+  if (dz == ntz - 1) dz = -1;
+  if (dz == 1 - ntz) dz = 1;
+  if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1) return kFar;
+  return (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
 }
 
-// The common case of every step is handled in straight-line code (LDS
-// gather, LDS charge tile, staying in the own bucket); the rare cases (a
-// particle outside the staged tile, a move beyond the charge halo, a full
-// destination bucket) sit behind wave-uniform branches taken only when some
-// lane of the wave needs them.
+__device__ __forceinline__ int neighbour_bucket(int b_own, int code, int ntx, int nty, int ntz,
+                                                int ntiles) {
+  int s = b_own / ntiles, o = b_own % ntiles;
+  int tx = wrapc(o % ntx + code % 3 - 1, ntx);
+  int ty = wrapc((o / ntx) % nty + (code / 3) % 3 - 1, nty);
+  int tz = wrapc(o / (ntx * nty) + code / 9 - 1, ntz);
+  return s * ntiles + (tz * nty + ty) * ntx + tx;
+}
+
+// slot of an arrival: a run at the front of bucket dest, or the spill region
+__device__ __forceinline__ int64_t arrival_slot(const FxArgs &a, int dest, int pos) {
+  int room = a.cap_out[dest] - a.count_in[dest];
+  if (pos < room) return a.start_out[dest] + pos;
+  int s = dest / a.ntiles;
+  int q = atomicAdd(&a.spill_out[s], 1);
+  if (q >= a.spill_cap) {
+    atomicOr(a.flag, kFlagSpillLost);
+    return -1;
+  }
+  return a.area + (int64_t)s * a.spill_cap + q;
+}
+
+// One particle.  The common case is straight-line code (LDS gather, LDS charge
+// tile, staying in the own bucket); the rare cases (outside the staged tile,
+// beyond the charge halo, outbox full) sit behind wave-uniform branches taken
+// only when some lane of the wave needs them.  Every lane stores exactly once
+// (lanes without a slot store to the trash slot), so no path leaves a store
+// count unknown to the wait-count analysis and the next particle's loads stay
+// in flight.
 template <int T, bool PUSH, bool DEPOSIT>
-__device__ __forceinline__ void fx_one(const FxArgs &a, int s, int b_own, bool act, Pcl p,
-                                       int ox, int oy, int oz, bool lds_ok, int64_t own_start,
-                                       int *n_stay, int *n_out, MoverRec *outbox,
-                                       const float4 *fA, const float2 *fB,
-                                       unsigned long long *rh, double ke[3]) {
+__device__ __forceinline__ void fx_one(const FxArgs &a, int s, int b_own, bool act, Pcl p, int ox,
+                                       int oy, int oz, bool lds_ok, int64_t own_end, int *n_stay,
+                                       int *n_out, OutRec *outbox, const float4 *fA,
+                                       const float2 *fB, unsigned long long *rh, double ke[3]) {
   constexpr int N3 = T + 3;
   const Grid &g = a.g;
   int ix = p.cell & kCellMask, iy = (p.cell >> kCellBits) & kCellMask, iz = p.cell >> (2 * kCellBits);
@@ -217,7 +273,7 @@ __device__ __forceinline__ void fx_one(const FxArgs &a, int s, int b_own, bool a
   }
   float fx = p.fx, fy = p.fy, fz = p.fz, vx = p.vx, vy = p.vy, vz = p.vz;
   if (PUSH) {
-    float f[6];
+    float f[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     int lx = ix - ox, ly = iy - oy, lz = iz - oz;
     bool in_tile = lds_ok && (unsigned)lx < (unsigned)T && (unsigned)ly < (unsigned)T &&
                    (unsigned)lz < (unsigned)T;
@@ -226,7 +282,7 @@ __device__ __forceinline__ void fx_one(const FxArgs &a, int s, int b_own, bool a
       ly = 0;
       lz = 0;
     }
-    gather_lds<T>(fA, fB, lx, ly, lz, fx, fy, fz, f);
+    if (lds_ok) gather_lds<T>(fA, fB, lx, ly, lz, fx, fy, fz, f);
     if (__ballot(act && !in_tile)) {
       if (act && !in_tile) gather_global(a.node32, g, ix, iy, iz, fx, fy, fz, f);
     }
@@ -237,9 +293,9 @@ __device__ __forceinline__ void fx_one(const FxArgs &a, int s, int b_own, bool a
       ke[2] += (double)vz * (double)vz;
     }
   }
-  move_axis(fx, ix, vx, a.gstep, g.nx, a.flag);
-  move_axis(fy, iy, vy, a.gstep, g.ny, a.flag);
-  move_axis(fz, iz, vz, a.gstep, g.nz, a.flag);
+  move_axis(fx, ix, vx, a.gstep, g.nx, a.flag, act);
+  move_axis(fy, iy, vy, a.gstep, g.ny, a.flag, act);
+  move_axis(fz, iz, vz, a.gstep, g.nz, a.flag, act);
   if (DEPOSIT) {
     int W[8];
     fixed_weights(fx, fy, fz, W);
@@ -268,13 +324,7 @@ __device__ __forceinline__ void fx_one(const FxArgs &a, int s, int b_own, bool a
       }
     }
   }
-  // ---- re-bucketing: destination bucket of the new cell.  Particles that
-  // stay take consecutive slots at the front of their own output bucket (LDS
-  // counter); the others go to the LDS outbox and k_fx_arrive appends them to
-  // their new bucket after every workgroup has finished.  No global atomic
-  // with a return value sits in this path, and every lane stores (a lane
-  // without a slot stores to the trash slot), so loads of the next particle
-  // stay in flight across the whole step.
+  // ---- re-bucketing
   int dest = s * a.ntiles + ((iz / T) * a.nty + (iy / T)) * a.ntx + (ix / T);
   bool stay = act && dest == b_own;
   int lane = threadIdx.x & 63;
@@ -286,46 +336,40 @@ __device__ __forceinline__ void fx_one(const FxArgs &a, int s, int b_own, bool a
     int base = 0;
     if (lane == leader) base = atomicAdd(n_stay, __popcll(m));
     base = __shfl(base, leader);
-    if (stay) dst = own_start + base + __popcll(m & lt);
+    if (stay) dst = own_end - 1 - (base + __popcll(m & lt));
   }
+  int cell = pack_cell(ix, iy, iz);
   uint64_t mv = __ballot(act && !stay);
   if (mv) {
     int leader = __ffsll((unsigned long long)mv) - 1;
     int base = 0;
     if (lane == leader) base = atomicAdd(n_out, __popcll(mv));
     base = __shfl(base, leader);
-    if (act && !stay) {
-      int k = base + __popcll(mv & lt);
-      MoverRec r;
+    int k = base + __popcll(mv & lt);
+    bool boxed = act && !stay && k < kOutbox;
+    if (boxed) {
+      OutRec r;
       r.dest = dest;
-      r.cell = pack_cell(ix, iy, iz);
+      r.cell = cell;
       r.pid = p.pid;
-      r.pad = 0;
+      r.code = 0;
+      r.rank = 0;
       r.fx = fx;
       r.fy = fy;
       r.fz = fz;
       r.vx = vx;
       r.vy = vy;
       r.vz = vz;
-      if (k < kOutbox) {
-        outbox[k] = r;
-      } else {  // outbox full: straight to the global mover list
-        int g2 = atomicAdd(a.mv_count, 1);
-        if (g2 < a.mv_cap)
-          a.mv[g2] = r;
-        else
-          atomicOr(a.flag, kFlagSpillLost);
+      outbox[k] = r;
+    }
+    if (__ballot(act && !stay && !boxed)) {  // outbox full: place directly
+      if (act && !stay && !boxed) {
+        int64_t d2 = arrival_slot(a, dest, atomicAdd(&a.back[dest], 1));
+        if (d2 >= 0) store_pcl(a.out, d2, cell, p.pid, fx, fy, fz, vx, vy, vz);
       }
     }
   }
-  a.out.cell[dst] = pack_cell(ix, iy, iz);
-  if (a.out.pid) a.out.pid[dst] = p.pid;
-  a.out.fx[dst] = fx;
-  a.out.fy[dst] = fy;
-  a.out.fz[dst] = fz;
-  a.out.vx[dst] = vx;
-  a.out.vy[dst] = vy;
-  a.out.vz[dst] = vz;
+  store_pcl(a.out, dst, cell, p.pid, fx, fy, fz, vx, vy, vz);
 }
 
 __device__ __forceinline__ void block_reduce3(double ke[3], double *out) {
@@ -360,9 +404,11 @@ __global__ __launch_bounds__(kBlock) void k_fx_step(FxArgs a) {
   __shared__ float4 fA[PUSH ? N1 * N1 * N1 : 1];
   __shared__ float2 fB[PUSH ? N1 * N1 * N1 : 1];
   __shared__ unsigned long long rh[DEPOSIT ? N3 * N3 * N3 : 1];
-  __shared__ MoverRec outbox[kOutbox];
+  __shared__ OutRec outbox[kOutbox];
   __shared__ int n_stay, n_out;
-  __shared__ int64_t out_base;
+  __shared__ int hist[kFar + 1], runbase[kFar];
+  __shared__ int64_t runstart[kFar];
+  __shared__ int runroom[kFar];
   double ke[3] = {0.0, 0.0, 0.0};
   const int bid = blockIdx.x;
   int part = bid;  // energy partial slot: bucket index, then spill blocks
@@ -371,6 +417,7 @@ __global__ __launch_bounds__(kBlock) void k_fx_step(FxArgs a) {
     n_stay = 0;
     n_out = 0;
   }
+  if (threadIdx.x <= kFar) hist[threadIdx.x] = 0;
   if (bid < a.nb) {
     b = bucket_of_block(bid, a.nb);
     part = b;
@@ -380,7 +427,9 @@ __global__ __launch_bounds__(kBlock) void k_fx_step(FxArgs a) {
     oy = ((t / a.ntx) % a.nty) * T;
     oz = (t / (a.ntx * a.nty)) * T;
     const int cnt = a.count_in[b];
+    const int narr = a.narr_in[b];
     const int64_t beg = a.start_in[b];
+    const int64_t gap = a.cap_in[b] - cnt;
     if (cnt > 0) {
       if (PUSH) {
         for (int q = threadIdx.x; q < 2 * N1 * N1 * N1; q += kBlock) {
@@ -400,16 +449,19 @@ __global__ __launch_bounds__(kBlock) void k_fx_step(FxArgs a) {
       if (DEPOSIT)
         for (int q = threadIdx.x; q < N3 * N3 * N3; q += kBlock) rh[q] = 0ull;
       __syncthreads();
-      const int64_t own_start = a.start_out[b];
+      const int64_t own_end = a.start_out[b] + a.cap_out[b];
       // software pipelined: the next particle's state is in flight while the
       // current one is pushed (the load index is clamped, so every lane loads)
       Pcl cur, nxt;
-      load_pcl(a.in, beg, min((int)threadIdx.x, cnt - 1), cur);
+      {
+        int i = min((int)threadIdx.x, cnt - 1);
+        load_pcl(a.in, beg + i + (i >= narr ? gap : 0), cur);
+      }
       for (int base = 0; base < cnt; base += kBlock) {
-        int inext = base + kBlock + (int)threadIdx.x;
-        load_pcl(a.in, beg, (unsigned)min(inext, cnt - 1), nxt);
+        int i = min(base + kBlock + (int)threadIdx.x, cnt - 1);
+        load_pcl(a.in, beg + i + (i >= narr ? gap : 0), nxt);
         fx_one<T, PUSH, DEPOSIT>(a, s, b, base + (int)threadIdx.x < cnt, cur, ox, oy, oz, true,
-                                 own_start, &n_stay, &n_out, outbox, fA, fB, rh, ke);
+                                 own_end, &n_stay, &n_out, outbox, fA, fB, rh, ke);
         cur = nxt;
       }
       if (DEPOSIT) {
@@ -440,59 +492,40 @@ __global__ __launch_bounds__(kBlock) void k_fx_step(FxArgs a) {
     for (int r = 0; r < rounds; r++) {
       int i = (r * kSpillBlocksPerSpecies + sub) * kBlock + threadIdx.x;
       Pcl cur{};
-      if (i < cnt) load_pcl(a.in, beg, (unsigned)i, cur);
+      if (i < cnt) load_pcl(a.in, beg + i, cur);
       fx_one<T, PUSH, DEPOSIT>(a, s, -1, i < cnt, cur, 0, 0, 0, false, 0, &n_stay, &n_out, outbox,
                                fA, fB, rh, ke);
     }
   }
-  // hand the outbox to the global mover list and publish the stayer count
+  // ---- place the outbox: one reserved run per destination bucket
   __syncthreads();
-  int nout = min(n_out, kOutbox);
-  if (threadIdx.x == 0) {
-    if (b >= 0) a.front[b] = n_stay;
-    out_base = nout > 0 ? (int64_t)atomicAdd(a.mv_count, nout) : 0;
+  const int nout = min(n_out, kOutbox);
+  for (int q = threadIdx.x; q < nout; q += kBlock) {
+    int c = dest_code(outbox[q].dest, b, a.ntx, a.nty, a.ntz, a.ntiles);
+    outbox[q].code = c;
+    if (c < kFar) outbox[q].rank = atomicAdd(&hist[c], 1);
   }
+  __syncthreads();
+  if (threadIdx.x < kFar && hist[threadIdx.x] > 0) {
+    int d = neighbour_bucket(b, threadIdx.x, a.ntx, a.nty, a.ntz, a.ntiles);
+    runbase[threadIdx.x] = atomicAdd(&a.back[d], hist[threadIdx.x]);
+    runstart[threadIdx.x] = a.start_out[d];
+    runroom[threadIdx.x] = a.cap_out[d] - a.count_in[d];
+  }
+  if (threadIdx.x == 0 && b >= 0) a.front[b] = n_stay;
   __syncthreads();
   for (int q = threadIdx.x; q < nout; q += kBlock) {
-    int64_t g2 = out_base + q;
-    if (g2 < a.mv_cap)
-      a.mv[g2] = outbox[q];
-    else
-      atomicOr(a.flag, kFlagSpillLost);
+    OutRec r = outbox[q];
+    int64_t dst;
+    if (r.code < kFar) {
+      int pos = runbase[r.code] + r.rank;
+      dst = pos < runroom[r.code] ? runstart[r.code] + pos : arrival_slot(a, r.dest, pos);
+    } else {
+      dst = arrival_slot(a, r.dest, atomicAdd(&a.back[r.dest], 1));
+    }
+    if (dst >= 0) store_pcl(a.out, dst, r.cell, r.pid, r.fx, r.fy, r.fz, r.vx, r.vy, r.vz);
   }
   if (PUSH) block_reduce3(ke, a.ke_part + 3 * (int64_t)part);
-}
-
-// movers of the step: appended behind the stayers of their destination bucket
-// (or, when that is full, to the species' spill region)
-__global__ __launch_bounds__(kBlock) void k_fx_arrive(FxArgs a) {
-  int n = *a.mv_count;
-  if (n > a.mv_cap) n = (int)a.mv_cap;
-  for (int k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) {
-    MoverRec r = a.mv[k];
-    int dest = r.dest;
-    int slot = a.front[dest] + atomicAdd(&a.back[dest], 1);
-    int64_t dst;
-    if (slot < a.cap_out[dest]) {
-      dst = a.start_out[dest] + slot;
-    } else {
-      int s = dest / a.ntiles;
-      int q = atomicAdd(&a.spill_out[s], 1);
-      if (q >= a.spill_cap) {
-        atomicOr(a.flag, kFlagSpillLost);
-        continue;
-      }
-      dst = a.area + (int64_t)s * a.spill_cap + q;
-    }
-    a.out.cell[dst] = r.cell;
-    if (a.out.pid) a.out.pid[dst] = r.pid;
-    a.out.fx[dst] = r.fx;
-    a.out.fy[dst] = r.fy;
-    a.out.fz[dst] = r.fz;
-    a.out.vx[dst] = r.vx;
-    a.out.vy[dst] = r.vy;
-    a.out.vz[dst] = r.vz;
-  }
 }
 
 // standalone charge deposition of the current positions (no move, no bucket change)
@@ -507,15 +540,15 @@ __global__ __launch_bounds__(kBlock) void k_fx_deposit(FxArgs a) {
     const int s = b / a.ntiles, t = b - s * a.ntiles;
     const int ox = (t % a.ntx) * T, oy = ((t / a.ntx) % a.nty) * T, oz = (t / (a.ntx * a.nty)) * T;
     const int cnt = a.count_in[b];
+    const int narr = a.narr_in[b];
     const int64_t beg = a.start_in[b];
+    const int64_t gap = a.cap_in[b] - cnt;
     if (cnt == 0) return;
     for (int q = threadIdx.x; q < N3 * N3 * N3; q += kBlock) rh[q] = 0ull;
     __syncthreads();
     unsigned long long *r = a.rho + (int64_t)s * a.ncell;
-    for (int base = 0; base < cnt; base += kBlock) {
-      int i = base + threadIdx.x;
-      if (i >= cnt) break;
-      int64_t src = beg + i;
+    for (int i = threadIdx.x; i < cnt; i += kBlock) {
+      int64_t src = beg + i + (i >= narr ? gap : 0);
       int cell = a.in.cell[src];
       int ix = cell & kCellMask, iy = (cell >> kCellBits) & kCellMask, iz = cell >> (2 * kCellBits);
       int W[8];
@@ -575,34 +608,33 @@ __global__ __launch_bounds__(kBlock) void k_fx_deposit(FxArgs a) {
   }
 }
 
-// output capacities of the next step: cap = count + count/4 + 64; clears the
-// output counters
-__global__ void k_fx_caps(const int32_t *count_in, int32_t *cap, int32_t *count_out,
-                          int32_t *spill_out, int nb, int nspec) {
+// output capacities of a step from the input counts; clears the spill counters
+__global__ void k_fx_caps(const int32_t *count_in, int32_t *cap_out, int32_t *spill_out, int nb,
+                          int nspec) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b < nb) {
     int c = count_in[b];
-    cap[b] = c + c / 4 + 64;
-    count_out[b] = 0;
+    cap_out[b] = c + c / 4 + 64;
   }
   if (b < nspec) spill_out[b] = 0;
 }
 
-// before a step: arrival counters and the mover list are empty
-__global__ void k_fx_step_prep(int32_t *back, int32_t *mv_count, int nb) {
+__global__ void k_fill_i32(int32_t *p, int n, int v) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b < nb) back[b] = 0;
-  if (b == 0) *mv_count = 0;
+  if (b < n) p[b] = v;
 }
 
-// Arrivals into a full bucket still bump its counter (the particle itself
-// goes to the spill region), so after a step the counters are clamped back to
-// the number of particles actually stored: count <= capacity, spill <= cap.
-__global__ void k_fx_clamp(int32_t *count, const int32_t *front, const int32_t *back,
-                           const int32_t *cap, int nb, int32_t *spill, int nspec,
-                           int32_t spill_cap) {
+// after a step: the output buckets' run lengths (arrivals that did not fit
+// were sent to the spill region), spill counters clamped
+__global__ void k_fx_finish(const int32_t *front, const int32_t *back, const int32_t *count_in,
+                            const int32_t *cap_out, int32_t *narr_out, int32_t *count_out, int nb,
+                            int32_t *spill, int nspec, int32_t spill_cap) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b < nb) count[b] = min(front[b] + back[b], cap[b]);
+  if (b < nb) {
+    int na = min(back[b], cap_out[b] - count_in[b]);
+    narr_out[b] = na;
+    count_out[b] = na + front[b];
+  }
   if (b < nspec) spill[b] = min(spill[b], spill_cap);
 }
 
@@ -614,22 +646,26 @@ struct RangeDesc {
   int ntiles;
   int rb;      // blocks per range
   const int64_t *bstart;
-  const int32_t *bcount;
+  const int32_t *bcount, *bnarr, *bcap;
   int64_t seg_start[EMG_MAX_SPECIES];
   const int32_t *seg_count_dev;        // device counts (spill) or nullptr
   int64_t seg_count[EMG_MAX_SPECIES];  // host counts (flat)
 };
 
 __device__ __forceinline__ void range_of(const RangeDesc &d, int r, int64_t &beg, int64_t &cnt,
-                                         int &s) {
+                                         int64_t &narr, int64_t &gap, int &s) {
   if (r < d.nrb) {
     beg = d.bstart[r];
     cnt = d.bcount[r];
+    narr = d.bnarr[r];
+    gap = d.bcap[r] - cnt;
     s = r / d.ntiles;
   } else {
     s = r - d.nrb;
     beg = d.seg_start[s];
     cnt = d.seg_count_dev ? (int64_t)d.seg_count_dev[s] : d.seg_count[s];
+    narr = cnt;
+    gap = 0;
   }
 }
 
@@ -643,11 +679,11 @@ template <int T>
 __global__ __launch_bounds__(kBlock) void k_fx_hist(RangeDesc d, StateFx in, int ntx, int nty,
                                                     int32_t *hist) {
   int r = blockIdx.x / d.rb, sub = blockIdx.x % d.rb;
-  int64_t beg, cnt;
+  int64_t beg, cnt, narr, gap;
   int s;
-  range_of(d, r, beg, cnt, s);
+  range_of(d, r, beg, cnt, narr, gap, s);
   for (int64_t i = (int64_t)sub * kBlock + threadIdx.x; i < cnt; i += (int64_t)d.rb * kBlock) {
-    int b = bucket_of_cell<T>(in.cell[beg + i], s, ntx, nty, d.ntiles);
+    int b = bucket_of_cell<T>(in.cell[beg + i + (i >= narr ? gap : 0)], s, ntx, nty, d.ntiles);
     atomicAdd(&hist[b], 1);
   }
 }
@@ -657,22 +693,16 @@ __global__ __launch_bounds__(kBlock) void k_fx_scatter(RangeDesc d, StateFx in, 
                                                        int ntx, int nty, const int64_t *start,
                                                        int32_t *cursor) {
   int r = blockIdx.x / d.rb, sub = blockIdx.x % d.rb;
-  int64_t beg, cnt;
+  int64_t beg, cnt, narr, gap;
   int s;
-  range_of(d, r, beg, cnt, s);
+  range_of(d, r, beg, cnt, narr, gap, s);
   for (int64_t i = (int64_t)sub * kBlock + threadIdx.x; i < cnt; i += (int64_t)d.rb * kBlock) {
-    int64_t src = beg + i;
+    int64_t src = beg + i + (i >= narr ? gap : 0);
     int cell = in.cell[src];
     int b = bucket_of_cell<T>(cell, s, ntx, nty, d.ntiles);
     int64_t dst = start[b] + atomicAdd(&cursor[b], 1);
-    out.cell[dst] = cell;
-    if (out.pid) out.pid[dst] = in.pid[src];
-    out.fx[dst] = in.fx[src];
-    out.fy[dst] = in.fy[src];
-    out.fz[dst] = in.fz[src];
-    out.vx[dst] = in.vx[src];
-    out.vy[dst] = in.vy[src];
-    out.vz[dst] = in.vz[src];
+    store_pcl(out, dst, cell, in.pid ? in.pid[src] : 0, in.fx[src], in.fy[src], in.fz[src],
+              in.vx[src], in.vy[src], in.vz[src]);
   }
 }
 
@@ -705,27 +735,22 @@ __global__ __launch_bounds__(kBlock) void k_fx_from64(State64 s64, SpeciesTab sp
     c[d] = ci;
     f[d] = fo;
   }
-  out.cell[m] = pack_cell(c[0], c[1], c[2]);
-  if (out.pid) out.pid[m] = s64.pid[m];
-  out.fx[m] = f[0];
-  out.fy[m] = f[1];
-  out.fz[m] = f[2];
-  out.vx[m] = (float)s64.vx[m];
-  out.vy[m] = (float)s64.vy[m];
-  out.vz[m] = (float)s64.vz[m];
+  store_pcl(out, m, pack_cell(c[0], c[1], c[2]), s64.pid[m], f[0], f[1], f[2], (float)s64.vx[m],
+            (float)s64.vy[m], (float)s64.vz[m]);
 }
 
 // bucketed FP32 -> FP64 species-contiguous (dst offset = flat scan of counts)
 __global__ __launch_bounds__(kBlock) void k_fx_to64(StateFx in, const int64_t *bstart,
-                                                    const int32_t *bcount, const int64_t *flat,
+                                                    const int32_t *bcount, const int32_t *bnarr,
+                                                    const int32_t *bcap, const int64_t *flat,
                                                     int ntiles, SpeciesTab sp, State64 out) {
   int b = blockIdx.x;
   int s = b / ntiles;
-  int cnt = bcount[b];
-  int64_t beg = bstart[b];
+  int cnt = bcount[b], narr = bnarr[b];
+  int64_t beg = bstart[b], gap = bcap[b] - cnt;
   int64_t base = sp.off[s] + (flat[b] - flat[s * ntiles]);
   for (int i = threadIdx.x; i < cnt; i += kBlock) {
-    int64_t src = beg + i, dst = base + i;
+    int64_t src = beg + i + (i >= narr ? gap : 0), dst = base + i;
     int cell = in.cell[src];
     int ix = cell & kCellMask, iy = (cell >> kCellBits) & kCellMask, iz = cell >> (2 * kCellBits);
     out.x[dst] = (double)ix + (double)in.fx[src];
@@ -740,23 +765,28 @@ __global__ __launch_bounds__(kBlock) void k_fx_to64(StateFx in, const int64_t *b
 
 // kinetic energy of the bucketed FP32 state (when no fused push partials exist)
 __global__ __launch_bounds__(kBlock) void k_fx_energy(StateFx in, const int64_t *bstart,
-                                                      const int32_t *bcount, int nb,
+                                                      const int32_t *bcount, const int32_t *bnarr,
+                                                      const int32_t *bcap, int nb,
                                                       const int32_t *spill, int64_t area,
                                                       int64_t spill_cap, double *part) {
   double ke[3] = {0.0, 0.0, 0.0};
   int bid = blockIdx.x;
-  int64_t beg;
-  int cnt;
+  int64_t beg, gap = 0;
+  int cnt, narr;
   if (bid < nb) {
     beg = bstart[bid];
     cnt = bcount[bid];
+    narr = bnarr[bid];
+    gap = bcap[bid] - cnt;
   } else {
     int s = bid - nb;
     beg = area + (int64_t)s * spill_cap;
     cnt = spill[s];
+    narr = cnt;
   }
   for (int i = threadIdx.x; i < cnt; i += kBlock) {
-    float vx = in.vx[beg + i], vy = in.vy[beg + i], vz = in.vz[beg + i];
+    int64_t src = beg + i + (i >= narr ? gap : 0);
+    float vx = in.vx[src], vy = in.vy[src], vz = in.vz[src];
     ke[0] += (double)vx * vx;
     ke[1] += (double)vy * vy;
     ke[2] += (double)vz * vz;
@@ -792,6 +822,7 @@ __global__ void k_copy_i32(const int32_t *src, int n, int32_t *dst) {
 
 FxArgs make_args(emg_ctx *c) {
   FxBuckets &B = c->fx;
+  int in = B.cur, out = 1 - B.cur;
   FxArgs a;
   a.g = c->g;
   a.ntx = B.ntx;
@@ -800,17 +831,21 @@ FxArgs make_args(emg_ctx *c) {
   a.ntiles = B.ntiles;
   a.nb = B.nb;
   a.nspec = c->sp.nspec;
-  a.in = B.st[B.cur];
-  a.out = B.st[1 - B.cur];
-  a.start_in = B.start[B.cur];
-  a.count_in = B.count[B.cur];
-  a.spill_in = B.spill[B.cur];
-  a.start_out = B.start[1 - B.cur];
-  a.count_out = B.count[1 - B.cur];
-  a.cap_out = B.capb;
-  a.spill_out = B.spill[1 - B.cur];
+  a.in = B.st[in];
+  a.out = B.st[out];
+  a.start_in = B.start[in];
+  a.count_in = B.count[in];
+  a.narr_in = B.narr[in];
+  a.cap_in = B.cap[in];
+  a.spill_in = B.spill[in];
+  a.start_out = B.start[out];
+  a.cap_out = B.cap[out];
+  a.front = B.front;
+  a.back = B.back;
+  a.spill_out = B.spill[out];
   a.area = B.area;
   a.spill_cap = B.spill_cap;
+  a.trash = B.trash;
   a.node32 = c->node32;
   a.rho = c->rhofx;
   a.ncell = c->ncell;
@@ -818,12 +853,6 @@ FxArgs make_args(emg_ctx *c) {
   a.gstep = 0.f;
   a.ke_part = c->ke_part;
   a.flag = c->dflag;
-  a.front = B.front;
-  a.back = B.back;
-  a.mv = (MoverRec *)B.mv;
-  a.mv_count = B.mv_count;
-  a.mv_cap = B.mv_cap;
-  a.trash = B.trash;
   return a;
 }
 
@@ -839,12 +868,13 @@ void step_t(emg_ctx *c, const FxArgs &a, bool push, bool deposit, unsigned grid)
     hipLaunchKernelGGL((k_fx_step<T, false, false>), dim3(grid), dim3(kBlock), 0, c->stream, a);
 }
 
-// bucket particles described by d (reading buffer 'src') into buffer 1 - src
+// bucket the particles described by d (reading buffer 'src') into buffer 1 - src
 int bucket_ranges(emg_ctx *c, RangeDesc d, int nr, int rb, int src) {
   FxBuckets &B = c->fx;
   int dst = 1 - src;
   d.rb = rb;
   unsigned grid = (unsigned)((int64_t)nr * rb);
+  unsigned gb = (unsigned)((B.nb + kBlock - 1) / kBlock);
   launch_zero(B.hist, sizeof(int32_t) * B.nb, c->stream);
   switch (B.tile) {
     case 4:
@@ -858,9 +888,10 @@ int bucket_ranges(emg_ctx *c, RangeDesc d, int nr, int rb, int src) {
     default: return set_error(EMG_ERR_ARG, "tile must be 4 or 8");
   }
   // capacities from the histogram, start offsets, cursor = 0, spill = 0
-  hipLaunchKernelGGL(k_fx_caps, dim3((B.nb + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream,
-                     B.hist, B.capb, B.cursor, B.spill[dst], B.nb, c->sp.nspec);
-  exclusive_scan_i32(B.capb, B.start[dst], B.nb, c->scan_tmp, c->stream);
+  hipLaunchKernelGGL(k_fx_caps, dim3(gb), dim3(kBlock), 0, c->stream, B.hist, B.cap[dst],
+                     B.spill[dst], B.nb, c->sp.nspec);
+  hipLaunchKernelGGL(k_fill_i32, dim3(gb), dim3(kBlock), 0, c->stream, B.cursor, B.nb, 0);
+  exclusive_scan_i32(B.cap[dst], B.start[dst], B.nb, c->scan_tmp, c->stream);
   switch (B.tile) {
     case 4:
       hipLaunchKernelGGL(k_fx_scatter<4>, dim3(grid), dim3(kBlock), 0, c->stream, d, B.st[src],
@@ -871,10 +902,26 @@ int bucket_ranges(emg_ctx *c, RangeDesc d, int nr, int rb, int src) {
                          B.st[dst], B.ntx, B.nty, B.start[dst], B.cursor);
       break;
   }
-  hipLaunchKernelGGL(k_copy_i32, dim3((B.nb + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream,
-                     B.cursor, B.nb, B.count[dst]);
+  // all particles of a bucket form one run from its start
+  hipLaunchKernelGGL(k_copy_i32, dim3(gb), dim3(kBlock), 0, c->stream, B.cursor, B.nb, B.count[dst]);
+  hipLaunchKernelGGL(k_copy_i32, dim3(gb), dim3(kBlock), 0, c->stream, B.cursor, B.nb, B.narr[dst]);
   B.cur = dst;
   return hip_check(hipGetLastError(), "bucketing");
+}
+
+RangeDesc bucket_desc(emg_ctx *c) {
+  FxBuckets &B = c->fx;
+  RangeDesc d{};
+  d.nrb = B.nb;
+  d.nspec = c->sp.nspec;
+  d.ntiles = B.ntiles;
+  d.bstart = B.start[B.cur];
+  d.bcount = B.count[B.cur];
+  d.bnarr = B.narr[B.cur];
+  d.bcap = B.cap[B.cur];
+  d.seg_count_dev = B.spill[B.cur];
+  for (int s = 0; s < c->sp.nspec; s++) d.seg_start[s] = B.area + (int64_t)s * B.spill_cap;
+  return d;
 }
 
 }  // namespace
@@ -900,7 +947,6 @@ int fx_alloc(emg_ctx *c, int64_t ntotal) {
   if (B.spill_cap > (1 << 30)) B.spill_cap = 1 << 30;
   B.cap_total = B.area + B.spill_cap * c->sp.nspec + 1;  // + the trash slot
   B.trash = B.cap_total - 1;
-  B.mv_cap = ntotal / 4 + 65536;
   for (int k = 0; k < 2; k++) {
     StateFx &S = B.st[k];
     void *p[7];
@@ -920,18 +966,18 @@ int fx_alloc(emg_ctx *c, int64_t ntotal) {
       return set_error(EMG_ERR_NOMEM, "FP32 pid buffers");
     if (hipMalloc(&B.start[k], sizeof(int64_t) * (B.nb + 1)) != hipSuccess ||
         hipMalloc(&B.count[k], sizeof(int32_t) * B.nb) != hipSuccess ||
+        hipMalloc(&B.narr[k], sizeof(int32_t) * B.nb) != hipSuccess ||
+        hipMalloc(&B.cap[k], sizeof(int32_t) * B.nb) != hipSuccess ||
         hipMalloc(&B.spill[k], sizeof(int32_t) * EMG_MAX_SPECIES) != hipSuccess)
       return set_error(EMG_ERR_NOMEM, "bucket tables");
     launch_zero(B.count[k], sizeof(int32_t) * B.nb, c->stream);
+    launch_zero(B.narr[k], sizeof(int32_t) * B.nb, c->stream);
+    launch_zero(B.cap[k], sizeof(int32_t) * B.nb, c->stream);
     launch_zero(B.spill[k], sizeof(int32_t) * EMG_MAX_SPECIES, c->stream);
     launch_zero(B.start[k], sizeof(int64_t) * (B.nb + 1), c->stream);
   }
   if (hipMalloc(&B.front, sizeof(int32_t) * B.nb) != hipSuccess ||
       hipMalloc(&B.back, sizeof(int32_t) * B.nb) != hipSuccess ||
-      hipMalloc(&B.mv, sizeof(MoverRec) * B.mv_cap) != hipSuccess ||
-      hipMalloc(&B.mv_count, sizeof(int32_t)) != hipSuccess)
-    return set_error(EMG_ERR_NOMEM, "mover list");
-  if (hipMalloc(&B.capb, sizeof(int32_t) * B.nb) != hipSuccess ||
       hipMalloc(&B.hist, sizeof(int32_t) * B.nb) != hipSuccess ||
       hipMalloc(&B.cursor, sizeof(int32_t) * B.nb) != hipSuccess)
     return set_error(EMG_ERR_NOMEM, "bucket tables");
@@ -962,9 +1008,6 @@ int fx_from64(emg_ctx *c, const State64 &src) {
   d.nrb = 0;
   d.nspec = c->sp.nspec;
   d.ntiles = B.ntiles;
-  d.bstart = nullptr;
-  d.bcount = nullptr;
-  d.seg_count_dev = nullptr;
   for (int s = 0; s < c->sp.nspec; s++) {
     d.seg_start[s] = c->sp.off[s];
     d.seg_count[s] = c->sp.cnt[s];
@@ -972,18 +1015,9 @@ int fx_from64(emg_ctx *c, const State64 &src) {
   return bucket_ranges(c, d, c->sp.nspec, 2048, flat);
 }
 
-// bucketed FP32 (with spill) -> compacted buckets (absorbs the spill regions)
+// bucketed FP32 (two runs per bucket, spill) -> one run per bucket, no spill
 int fx_rebucket(emg_ctx *c) {
-  FxBuckets &B = c->fx;
-  RangeDesc d{};
-  d.nrb = B.nb;
-  d.nspec = c->sp.nspec;
-  d.ntiles = B.ntiles;
-  d.bstart = B.start[B.cur];
-  d.bcount = B.count[B.cur];
-  d.seg_count_dev = B.spill[B.cur];
-  for (int s = 0; s < c->sp.nspec; s++) d.seg_start[s] = B.area + (int64_t)s * B.spill_cap;
-  return bucket_ranges(c, d, B.nb + c->sp.nspec, 1, B.cur);
+  return bucket_ranges(c, bucket_desc(c), c->fx.nb + c->sp.nspec, 1, c->fx.cur);
 }
 
 int fx_to64(emg_ctx *c, State64 &dst) {
@@ -991,21 +1025,22 @@ int fx_to64(emg_ctx *c, State64 &dst) {
   // spill regions are absorbed first so every particle sits in a bucket
   int rc = fx_rebucket(c);
   if (rc) return rc;
-  // flat offsets of the buckets
   exclusive_scan_i32(B.count[B.cur], B.start[1 - B.cur], B.nb, c->scan_tmp, c->stream);
   hipLaunchKernelGGL(k_fx_to64, dim3(B.nb), dim3(kBlock), 0, c->stream, B.st[B.cur],
-                     B.start[B.cur], B.count[B.cur], B.start[1 - B.cur], B.ntiles, c->sp, dst);
+                     B.start[B.cur], B.count[B.cur], B.narr[B.cur], B.cap[B.cur],
+                     B.start[1 - B.cur], B.ntiles, c->sp, dst);
   return hip_check(hipGetLastError(), "fx_to64");
 }
 
 void launch_fx_step(emg_ctx *c, int ustep, bool push, bool deposit, bool em) {
   (void)em;
   FxBuckets &B = c->fx;
+  int in = B.cur, out = 1 - B.cur;
   unsigned gb = (unsigned)((B.nb + kBlock - 1) / kBlock);
-  hipLaunchKernelGGL(k_fx_caps, dim3(gb), dim3(kBlock), 0, c->stream, B.count[B.cur], B.capb,
-                     B.count[1 - B.cur], B.spill[1 - B.cur], B.nb, c->sp.nspec);
-  hipLaunchKernelGGL(k_fx_step_prep, dim3(gb), dim3(kBlock), 0, c->stream, B.back, B.mv_count, B.nb);
-  exclusive_scan_i32(B.capb, B.start[1 - B.cur], B.nb, c->scan_tmp, c->stream);
+  hipLaunchKernelGGL(k_fx_caps, dim3(gb), dim3(kBlock), 0, c->stream, B.count[in], B.cap[out],
+                     B.spill[out], B.nb, c->sp.nspec);
+  hipLaunchKernelGGL(k_fill_i32, dim3(gb), dim3(kBlock), 0, c->stream, B.back, B.nb, 0);
+  exclusive_scan_i32(B.cap[out], B.start[out], B.nb, c->scan_tmp, c->stream);
   if (deposit) launch_zero(c->rhofx, sizeof(unsigned long long) * c->ncell * c->sp.nspec, c->stream);
   FxArgs a = make_args(c);
   a.gstep = (float)ustep;
@@ -1016,10 +1051,10 @@ void launch_fx_step(emg_ctx *c, int ustep, bool push, bool deposit, bool em) {
     case 8: step_t<8>(c, a, push, deposit, grid); break;
   }
   timer_end(c, kTPush);
-  hipLaunchKernelGGL(k_fx_arrive, dim3(512), dim3(kBlock), 0, c->stream, a);
-  hipLaunchKernelGGL(k_fx_clamp, dim3(gb), dim3(kBlock), 0, c->stream, B.count[1 - B.cur], B.front,
-                     B.back, B.capb, B.nb, B.spill[1 - B.cur], c->sp.nspec, (int32_t)B.spill_cap);
-  B.cur = 1 - B.cur;
+  hipLaunchKernelGGL(k_fx_finish, dim3(gb), dim3(kBlock), 0, c->stream, B.front, B.back,
+                     B.count[in], B.cap[out], B.narr[out], B.count[out], B.nb, B.spill[out],
+                     c->sp.nspec, (int32_t)B.spill_cap);
+  B.cur = out;
 }
 
 void launch_fx_deposit(emg_ctx *c) {
@@ -1039,8 +1074,8 @@ void launch_fx_energy(emg_ctx *c) {
   FxBuckets &B = c->fx;
   timer_begin(c, kTEnergy);
   hipLaunchKernelGGL(k_fx_energy, dim3(B.nb + c->sp.nspec), dim3(kBlock), 0, c->stream,
-                     B.st[B.cur], B.start[B.cur], B.count[B.cur], B.nb, B.spill[B.cur], B.area,
-                     B.spill_cap, c->ke_part);
+                     B.st[B.cur], B.start[B.cur], B.count[B.cur], B.narr[B.cur], B.cap[B.cur], B.nb,
+                     B.spill[B.cur], B.area, B.spill_cap, c->ke_part);
   hipLaunchKernelGGL(k_ke_final, dim3(c->sp.nspec), dim3(kBlock), 0, c->stream, c->ke_part,
                      B.ntiles, B.nb, 1, c->ke_out);
   timer_end(c, kTEnergy);
