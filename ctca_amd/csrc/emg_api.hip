@@ -327,11 +327,13 @@ int emg_destroy(emg_ctx *c) {
     dfree(B.start[k]);
     dfree(B.count[k]);
     dfree(B.spill[k]);
-    dfree(B.narr[k]);
+    dfree(B.nst[k]);
     dfree(B.cap[k]);
+    dfree(B.ob[k]);
+    dfree(B.pre[k]);
   }
-  dfree(B.front);
-  dfree(B.back);
+  dfree(B.stay);
+  dfree(B.extra);
   dfree(B.hist);
   dfree(B.cursor);
   dfree(c->eb_dev);

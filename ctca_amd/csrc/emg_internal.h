@@ -52,6 +52,13 @@ struct StateFx {
   int32_t *pid;   // nullptr unless the context tracks pid
 };
 
+// One mover in a bucket's outbox (32 bytes, two 16-byte halves)
+struct alignas(16) FxRec {
+  int32_t cell;
+  float fx, fy, fz, vx, vy, vz;
+  int32_t pid;
+};
+
 struct FxBuckets {
   int tile;            // tile edge T
   int ntx, nty, ntz;   // tiles per axis
@@ -62,14 +69,16 @@ struct FxBuckets {
   int64_t spill_cap;
   int64_t *start[2];   // [nb + 1] first slot of each bucket
   int32_t *count[2];   // [nb] particles in each bucket
-  int32_t *narr[2];    // [nb] of which arrivals (front run); the rest are stayers (back run)
+  int32_t *nst[2];     // [nb] of which in the first run (stayers); the rest (direct
+                       //      arrivals) sit at the end of the bucket's slots
   int32_t *cap[2];     // [nb] slots owned by each bucket
   int32_t *hist;       // [nb] bucketing scratch: histogram
   int32_t *cursor;     // [nb] bucketing scratch: scatter cursors
   int32_t *spill[2];   // [nspec] spill counters
-  int32_t *front;      // [nb] stayers written by the bucket's workgroup (this step)
-  int32_t *back;       // [nb] arrivals reserved by other workgroups (this step)
-  int64_t trash;       // slot index that takes the stores of lanes without a slot
+  int32_t *stay;       // [nb] stayers written by the bucket's workgroup (this step)
+  int32_t *extra;      // [nb] direct arrivals reserved by atomics (this step)
+  FxRec *ob[2];        // [nb * outbox] movers of each bucket, grouped by direction
+  int32_t *pre[2];     // [nb * 28] exclusive prefix of the outbox direction counts
   StateFx st[2];
   int cur;             // buffer holding the live particles
 };

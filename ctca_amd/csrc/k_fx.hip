@@ -5,38 +5,47 @@
 // tile being a T^3 block of cells.  Structure of arrays per buffer: packed
 // cell ix | iy << 10 | iz << 20, offsets fx, fy, fz in [0, 1), velocities.
 // Bucket b of a buffer owns the slots [start[b], start[b] + cap[b]) and holds
-// count[b] particles in two runs: narr[b] particles that arrived from other
-// buckets at the front, count[b] - narr[b] particles that stayed at the back:
-//   slot(i) = start + i                      for i <  narr
-//   slot(i) = start + cap - count + i        for i >= narr
+// count[b] particles in two runs: nst[b] stayers from the front, and the
+// direct arrivals (normally none) at the very end:
+//   slot(i) = start + i                      for i <  nst
+//   slot(i) = start + cap - count + i        for i >= nst
+// Besides its slots, every bucket has an outbox: the particles that left the
+// bucket's tile for one of the 26 neighbouring tiles in the last step, stored
+// contiguously as 32-byte records grouped by direction, with the exclusive
+// prefix of the 27 direction counts in pre[b][0..27].  Those particles belong
+// to the neighbours: each bucket pulls its arrivals from its 26 neighbours'
+// outboxes at the start of the next step.
 //
-// A step reads buffer cur and writes buffer 1 - cur (cap = count + count/4 +
-// 64 per bucket, start = exclusive scan of cap):
-//   * one workgroup per bucket stages the T+1 nodes per axis of the bucket's
-//     relocated fields into LDS, pushes every particle (FP32 Boris), moves it
-//     (cell + offset), deposits its charge into an LDS tile of T+3 nodes per
-//     axis as integer CIC weights (2^24 per unit weight, the eight weights of
-//     a particle summing to exactly 2^24), and writes it back:
-//       - a particle that stays in the tile takes the next slot from the back
-//         of its own output bucket (LDS counter, no global atomic);
-//       - a particle that changes tile goes to an LDS outbox; at the end of
-//         the workgroup the outbox is grouped by destination (the 26
-//         neighbouring tiles), one atomic per destination reserves a run at
-//         the front of that bucket, and the run is written contiguously;
+// A step reads buffer cur and writes buffer 1 - cur.  One workgroup per bucket:
+//   * stages the T+1 nodes per axis of the relocated fields into LDS;
+//   * pushes (FP32 Boris), moves (cell + offset) and deposits (integer CIC
+//     weights, 2^24 per unit weight, the eight weights of a particle summing
+//     to exactly 2^24, into an LDS tile of T+3 nodes per axis) its own
+//     particles and then the arrivals pulled from the neighbours' outboxes;
+//   * a particle that stays in the tile takes the next slot of the bucket's
+//     first run (LDS counter); a particle that leaves for a neighbouring tile
+//     goes to the LDS outbox, which the workgroup writes out grouped by
+//     direction at the end -- every global write of the step is a contiguous
+//     run, none is a few scattered bytes in another bucket's cache lines;
+//   * a particle that moves further than a neighbouring tile (or overflows the
+//     outbox) is placed directly: one of kExtra slots at the end of the
+//     destination bucket (global atomic), else the species' spill region,
+//     which the next step processes through the global-memory path;
 //   * the charge tile is added to the species' global int64 accumulator with
 //     one atomic per non-zero node.  Integer sums are associative, so rho is
 //     bitwise independent of particle order, scheduling and launch shape.
-// Arrivals beyond a bucket's room (cap - previous count) and particles that
-// moved further than a neighbouring tile go through a per-species spill
-// region that the next step processes through the global-memory path.
 #include "emg_internal.h"
 
 namespace emg {
 namespace {
 
-constexpr int kSpillBlocksPerSpecies = 8;
-constexpr int kOutbox = 256;   // movers staged in LDS per workgroup
-constexpr int kFar = 27;       // outbox code of a destination that is not a neighbour
+constexpr int kSpillBlocksPerSpecies = 64;
+constexpr int kOutbox = 512;  // movers staged per bucket (LDS and global outbox)
+constexpr int kExtra = 16;    // slots per bucket for direct arrivals
+constexpr int kCodes = 27;    // direction codes (dz+1)*9 + (dy+1)*3 + (dx+1); 13 = same tile
+constexpr int kFar = 27;      // not a neighbouring tile
+constexpr int kPre = 28;      // prefix table entries per bucket
+constexpr uint32_t kNoStore = 0x80000000u;  // buffer offset past every array: store dropped
 
 __device__ __forceinline__ int wrapc(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
@@ -47,17 +56,22 @@ __device__ __forceinline__ int pack_cell(int ix, int iy, int iz) {
 struct FxArgs {
   Grid g;
   int ntx, nty, ntz, ntiles, nb, nspec;
+  int near_ok;  // at least three tiles per axis: the 26 neighbours are distinct
   StateFx in, out;
   // input buffer
   const int64_t *start_in;
-  const int32_t *count_in, *narr_in, *cap_in, *spill_in;
+  const int32_t *count_in, *nst_in, *cap_in, *spill_in, *pre_in;
+  const FxRec *ob_in;
   // output buffer
   const int64_t *start_out;
   const int32_t *cap_out;
-  int32_t *front;  // stayers per bucket (written by the bucket's workgroup)
-  int32_t *back;   // arrivals per bucket (atomic)
+  int32_t *stay_out;  // stayers per bucket (written by the bucket's workgroup)
+  int32_t *extra;     // direct arrivals per bucket (atomic)
   int32_t *spill_out;
-  int64_t area, spill_cap, trash;
+  int32_t *pre_out;
+  FxRec *ob_out;
+  int64_t area, spill_cap;
+  int32_t nrec;  // bytes per particle array of a buffer (bound of the masked stores)
   const float *node32;
   unsigned long long *rho;
   int64_t ncell;
@@ -72,11 +86,6 @@ struct Pcl {
   float fx, fy, fz, vx, vy, vz;
 };
 
-struct OutRec {
-  int dest, cell, pid, code, rank;
-  float fx, fy, fz, vx, vy, vz;
-};
-
 __device__ __forceinline__ void load_pcl(const StateFx &in, int64_t slot, Pcl &p) {
   p.cell = in.cell[slot];
   p.pid = in.pid ? in.pid[slot] : 0;
@@ -86,6 +95,61 @@ __device__ __forceinline__ void load_pcl(const StateFx &in, int64_t slot, Pcl &p
   p.vx = in.vx[slot];
   p.vy = in.vy[slot];
   p.vz = in.vz[slot];
+}
+
+// The hot loop addresses every particle array with one 32-bit byte offset
+// (slot * 4, slots < 2^29) added to the array's uniform base, so each load
+// needs no per-array 64-bit address arithmetic.
+template <class U>
+__device__ __forceinline__ U &at(U *base, uint32_t off) {
+  return *reinterpret_cast<U *>(reinterpret_cast<char *>(base) + off);
+}
+
+template <bool PID>
+__device__ __forceinline__ void load_off(const StateFx &in, uint32_t off, Pcl &p) {
+  p.cell = at(in.cell, off);
+  p.pid = PID ? at(in.pid, off) : 0;
+  p.fx = at(in.fx, off);
+  p.fy = at(in.fy, off);
+  p.fz = at(in.fz, off);
+  p.vx = at(in.vx, off);
+  p.vy = at(in.vy, off);
+  p.vz = at(in.vz, off);
+}
+
+__device__ __forceinline__ void load_rec(const FxRec *ob, int64_t k, Pcl &p) {
+  const float4 *q = reinterpret_cast<const float4 *>(ob + k);
+  float4 h0 = q[0], h1 = q[1];
+  p.cell = __float_as_int(h0.x);
+  p.fx = h0.y;
+  p.fy = h0.z;
+  p.fz = h0.w;
+  p.vx = h1.x;
+  p.vy = h1.y;
+  p.vz = h1.z;
+  p.pid = __float_as_int(h1.w);
+}
+
+// Masked store of one value per lane through a buffer resource: a lane whose
+// offset is kNoStore lies outside the buffer and its store is dropped by the
+// hardware range check, so the instruction count never depends on the data.
+__device__ __forceinline__ void bstore(void *base, int32_t nrec, uint32_t off, uint32_t v) {
+  __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(base, 0, nrec, 0x00020000);
+  __builtin_amdgcn_raw_buffer_store_b32(v, r, (int)off, 0, 0);
+}
+
+template <bool PID>
+__device__ __forceinline__ void store_masked(const StateFx &out, int32_t nrec, uint32_t off,
+                                             int cell, int pid, float fx, float fy, float fz,
+                                             float vx, float vy, float vz) {
+  bstore(out.cell, nrec, off, (uint32_t)cell);
+  if (PID) bstore(out.pid, nrec, off, (uint32_t)pid);
+  bstore(out.fx, nrec, off, __float_as_uint(fx));
+  bstore(out.fy, nrec, off, __float_as_uint(fy));
+  bstore(out.fz, nrec, off, __float_as_uint(fz));
+  bstore(out.vx, nrec, off, __float_as_uint(vx));
+  bstore(out.vy, nrec, off, __float_as_uint(vy));
+  bstore(out.vz, nrec, off, __float_as_uint(vz));
 }
 
 __device__ __forceinline__ void store_pcl(const StateFx &out, int64_t dst, int cell, int pid,
@@ -128,26 +192,38 @@ __device__ __forceinline__ void gather_global(const float *__restrict__ node32, 
 // trilinear gather from the LDS field tile, two corners (one x pair) at a
 // time: bounds the registers held by LDS reads
 template <int T>
-__device__ __forceinline__ void gather_lds(const float4 *fA, const float2 *fB, int lx, int ly,
-                                           int lz, float fx, float fy, float fz, float f[6]) {
+__device__ __forceinline__ void gather_lds(const float4 *fA, const float2 *fB, int q0, float fx,
+                                           float fy, float fz, float f[6]) {
   constexpr int N1 = T + 1;
   float wx0 = 1.f - fx;
   float wyz[4] = {(1.f - fy) * (1.f - fz), fy * (1.f - fz), (1.f - fy) * fz, fy * fz};
-  int q0 = (lz * N1 + ly) * N1 + lx;
-#pragma unroll
-  for (int c = 0; c < 6; c++) f[c] = 0.f;
 #pragma unroll
   for (int r = 0; r < 4; r++) {
     int q = q0 + ((r >> 1) * N1 + (r & 1)) * N1;
     float4 A0 = fA[q], A1 = fA[q + 1];
     float2 B0 = fB[q], B1 = fB[q + 1];
     float w0 = wx0 * wyz[r], w1 = fx * wyz[r];
-    f[0] += w0 * A0.x + w1 * A1.x;
-    f[1] += w0 * A0.y + w1 * A1.y;
-    f[2] += w0 * A0.z + w1 * A1.z;
-    f[3] += w0 * A0.w + w1 * A1.w;
-    f[4] += w0 * B0.x + w1 * B1.x;
-    f[5] += w0 * B0.y + w1 * B1.y;
+    if (r == 0) {
+      f[0] = w0 * A0.x;
+      f[1] = w0 * A0.y;
+      f[2] = w0 * A0.z;
+      f[3] = w0 * A0.w;
+      f[4] = w0 * B0.x;
+      f[5] = w0 * B0.y;
+    } else {
+      f[0] = fmaf(w0, A0.x, f[0]);
+      f[1] = fmaf(w0, A0.y, f[1]);
+      f[2] = fmaf(w0, A0.z, f[2]);
+      f[3] = fmaf(w0, A0.w, f[3]);
+      f[4] = fmaf(w0, B0.x, f[4]);
+      f[5] = fmaf(w0, B0.y, f[5]);
+    }
+    f[0] = fmaf(w1, A1.x, f[0]);
+    f[1] = fmaf(w1, A1.y, f[1]);
+    f[2] = fmaf(w1, A1.z, f[2]);
+    f[3] = fmaf(w1, A1.w, f[3]);
+    f[4] = fmaf(w1, B1.x, f[4]);
+    f[5] = fmaf(w1, B1.y, f[5]);
     __builtin_amdgcn_sched_barrier(0);
   }
 }
@@ -172,49 +248,56 @@ __device__ __forceinline__ void boris32(float &vx, float &vy, float &vz, const f
   vz += ez;
 }
 
-// move one axis: offset += v * dt, carry whole cells into the index, periodic
-__device__ __forceinline__ void move_axis(float &f, int &i, float v, float dt, int n, int *flag,
-                                          bool act) {
-  float nf = f + v * dt;
+constexpr float kBelowOne = 0x1.fffffep-1f;
+
+// move one axis: offset += v * dt, whole cells carried into the index.  Returns
+// the cell displacement; the offset stays in [0, 1) (an offset that rounds up
+// to 1.0 is held at the largest float below 1, one ulp from the cell face).
+__device__ __forceinline__ int move_offset(float &f, float v, float dt) {
+  float nf = fmaf(v, dt, f);
   float fl = floorf(nf);
-  int d = (int)fl;
-  nf -= fl;
-  if (nf >= 1.f) {  // a tiny negative offset rounds to 1 - eps -> 1.0
-    nf -= 1.f;
-    d += 1;
-  }
-  int ni = i + d;
-  if (ni < 0) ni += n;
-  else if (ni >= n) ni -= n;
-  if ((unsigned)ni >= (unsigned)n) {  // moved more than one box length, or NaN
-    if (act) atomicOr(flag, kFlagFar);
-    ni = ni < 0 ? 0 : n - 1;
-  }
-  if (!(nf >= 0.f && nf < 1.f)) nf = 0.f;
-  f = nf;
-  i = ni;
+  f = fminf(nf - fl, kBelowOne);
+  return (int)fl;
 }
 
-// integer CIC weights summing to exactly 2^24
+// periodic index i + d for i in [0, n), d in [-n, n]
+__device__ __forceinline__ int wrap_add(int i, int d, int n) {
+  unsigned x = (unsigned)(i + d + n);
+  return (int)min(min(x, x - (unsigned)n), x - 2u * (unsigned)n);
+}
+
+// |d| >= n: a move of at least one box length
+__device__ __forceinline__ bool far_move(int d, int n) {
+  return (unsigned)(d + (n - 1)) > (unsigned)(2 * n - 2);
+}
+
+// integer CIC weights: seven truncated products, the eighth takes the
+// remainder, so the eight sum to exactly 2^24
 __device__ __forceinline__ void fixed_weights(float fx, float fy, float fz, int W[8]) {
-  float wx[2] = {1.f - fx, fx};
-  float wyz[4] = {(1.f - fy) * (1.f - fz) * 16777216.f, fy * (1.f - fz) * 16777216.f,
-                  (1.f - fy) * fz * 16777216.f, fy * fz * 16777216.f};
-  int sum = 0;
-#pragma unroll
-  for (int q = 0; q < 7; q++) {
-    W[q] = (int)rintf(wx[q & 1] * wyz[q >> 1]);
-    sum += W[q];
-  }
-  W[7] = (1 << kFixShift) - sum;
+  float gx = 1.f - fx;
+  float z0 = (1.f - fz) * 16777216.f, z1 = fz * 16777216.f;
+  float w00 = (1.f - fy) * z0, w10 = fy * z0, w01 = (1.f - fy) * z1, w11 = fy * z1;
+  W[0] = (int)(gx * w00);
+  W[1] = (int)(fx * w00);
+  W[2] = (int)(gx * w10);
+  W[3] = (int)(fx * w10);
+  W[4] = (int)(gx * w01);
+  W[5] = (int)(fx * w01);
+  W[6] = (int)(gx * w11);
+  W[7] = (1 << kFixShift) - ((W[0] + W[1] + W[2]) + (W[3] + W[4]) + (W[5] + W[6]));
 }
 
-// destination bucket -> outbox code (26 neighbouring tiles) or kFar
-__device__ __forceinline__ int dest_code(int dest, int b_own, int ntx, int nty, int ntz,
-                                         int ntiles) {
-  if (b_own < 0 || ntx < 3 || nty < 3 || ntz < 3) return kFar;
-  if (dest / ntiles != b_own / ntiles) return kFar;
-  int t = dest % ntiles, o = b_own % ntiles;
+template <int T>
+__device__ __forceinline__ int tile_of_cell(int cell, int ntx, int nty) {
+  constexpr int kLog2T = T == 4 ? 2 : 3;
+  int ix = cell & kCellMask, iy = (cell >> kCellBits) & kCellMask,
+      iz = (int)((unsigned)cell >> (2 * kCellBits));
+  return ((iz >> kLog2T) * nty + (iy >> kLog2T)) * ntx + (ix >> kLog2T);
+}
+
+// direction code of tile t seen from tile o, or kFar
+__device__ __forceinline__ int dir_code(int t, int o, int ntx, int nty, int ntz, int near_ok) {
+  if (!near_ok) return kFar;
   int dx = t % ntx - o % ntx, dy = (t / ntx) % nty - (o / ntx) % nty,
       dz = t / (ntx * nty) - o / (ntx * nty);
   if (dx == ntx - 1) dx = -1;
@@ -227,87 +310,115 @@ __device__ __forceinline__ int dest_code(int dest, int b_own, int ntx, int nty, 
   return (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
 }
 
-__device__ __forceinline__ int neighbour_bucket(int b_own, int code, int ntx, int nty, int ntz,
+// the bucket of the same species whose tile lies in direction `code` of b's tile
+__device__ __forceinline__ int neighbour_bucket(int b, int code, int ntx, int nty, int ntz,
                                                 int ntiles) {
-  int s = b_own / ntiles, o = b_own % ntiles;
+  int s = b / ntiles, o = b % ntiles;
   int tx = wrapc(o % ntx + code % 3 - 1, ntx);
   int ty = wrapc((o / ntx) % nty + (code / 3) % 3 - 1, nty);
   int tz = wrapc(o / (ntx * nty) + code / 9 - 1, ntz);
   return s * ntiles + (tz * nty + ty) * ntx + tx;
 }
 
-// slot of an arrival: a run at the front of bucket dest, or the spill region
-__device__ __forceinline__ int64_t arrival_slot(const FxArgs &a, int dest, int pos) {
-  int room = a.cap_out[dest] - a.count_in[dest];
-  if (pos < room) return a.start_out[dest] + pos;
-  int s = dest / a.ntiles;
-  int q = atomicAdd(&a.spill_out[s], 1);
-  if (q >= a.spill_cap) {
-    atomicOr(a.flag, kFlagSpillLost);
-    return -1;
+// place one particle into bucket dest by a global atomic: one of the bucket's
+// kExtra end slots, else the species' spill region
+__device__ void place_direct(const FxArgs &a, int dest, int cell, int pid, float fx, float fy,
+                             float fz, float vx, float vy, float vz) {
+  int idx = atomicAdd(&a.extra[dest], 1);
+  int64_t slot;
+  if (idx < kExtra) {
+    slot = a.start_out[dest] + a.cap_out[dest] - 1 - idx;
+  } else {
+    int s = dest / a.ntiles;
+    int q = atomicAdd(&a.spill_out[s], 1);
+    if (q >= a.spill_cap) {
+      atomicOr(a.flag, kFlagSpillLost);
+      return;
+    }
+    slot = a.area + (int64_t)s * a.spill_cap + q;
   }
-  return a.area + (int64_t)s * a.spill_cap + q;
+  store_pcl(a.out, slot, cell, pid, fx, fy, fz, vx, vy, vz);
 }
 
 // One particle.  The common case is straight-line code (LDS gather, LDS charge
-// tile, staying in the own bucket); the rare cases (outside the staged tile,
-// beyond the charge halo, outbox full) sit behind wave-uniform branches taken
-// only when some lane of the wave needs them.  Every lane stores exactly once
-// (lanes without a slot store to the trash slot), so no path leaves a store
-// count unknown to the wait-count analysis and the next particle's loads stay
-// in flight.
-template <int T, bool PUSH, bool DEPOSIT>
-__device__ __forceinline__ void fx_one(const FxArgs &a, int s, int b_own, bool act, Pcl p, int ox,
-                                       int oy, int oz, bool lds_ok, int64_t own_end, int *n_stay,
-                                       int *n_out, OutRec *outbox, const float4 *fA,
-                                       const float2 *fB, unsigned long long *rh, double ke[3]) {
-  constexpr int N3 = T + 3;
+// tile, staying in the own bucket or going to the LDS outbox); the rare cases
+// (outside the staged tile, beyond the charge halo, a move of a box length or
+// more, outbox full) sit behind wave-uniform branches taken only when some
+// lane of the wave needs them.  The stayers' stores are masked stores, so the
+// common path issues the same memory instructions whatever the data.  `act`
+// is false for lanes past the end of a round; full rounds pass a literal true.
+template <int T, bool PUSH, bool DEPOSIT, bool PID>
+__device__ __forceinline__ void fx_one(const FxArgs &a, int s, bool act, const Pcl &p, int ox,
+                                       int oy, int oz, bool lds_ok, uint32_t stay_base,
+                                       int *n_stay, int *n_out, FxRec *outbox, const float4 *fA,
+                                       const float2 *fB, unsigned long long *rh, float ke[3]) {
+  constexpr int N1 = T + 1, N3 = T + 3;
   const Grid &g = a.g;
-  int ix = p.cell & kCellMask, iy = (p.cell >> kCellBits) & kCellMask, iz = p.cell >> (2 * kCellBits);
-  if ((unsigned)ix >= (unsigned)g.nx || (unsigned)iy >= (unsigned)g.ny ||
-      (unsigned)iz >= (unsigned)g.nz) {  // never for a stored particle; keeps indexing in bounds
-    if (act) atomicOr(a.flag, kFlagDomain);
-    ix = 0;
-    iy = 0;
-    iz = 0;
-  }
+  int ix = p.cell & kCellMask, iy = (p.cell >> kCellBits) & kCellMask,
+      iz = (int)((unsigned)p.cell >> (2 * kCellBits));
+  int lx = ix - ox, ly = iy - oy, lz = iz - oz;  // cell in the staged tile
+  bool in_tile = lds_ok && max((unsigned)lx, max((unsigned)ly, (unsigned)lz)) < (unsigned)T;
   float fx = p.fx, fy = p.fy, fz = p.fz, vx = p.vx, vy = p.vy, vz = p.vz;
-  if (PUSH) {
-    float f[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    int lx = ix - ox, ly = iy - oy, lz = iz - oz;
-    bool in_tile = lds_ok && (unsigned)lx < (unsigned)T && (unsigned)ly < (unsigned)T &&
-                   (unsigned)lz < (unsigned)T;
+  float f[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  if (PUSH && lds_ok)
+    gather_lds<T>(fA, fB, in_tile ? (lz * N1 + ly) * N1 + lx : 0, fx, fy, fz, f);
+  if (__ballot(!in_tile)) {
     if (!in_tile) {
-      lx = 0;
-      ly = 0;
-      lz = 0;
-    }
-    if (lds_ok) gather_lds<T>(fA, fB, lx, ly, lz, fx, fy, fz, f);
-    if (__ballot(act && !in_tile)) {
-      if (act && !in_tile) gather_global(a.node32, g, ix, iy, iz, fx, fy, fz, f);
-    }
-    boris32(vx, vy, vz, f, a.qm[s] * a.gstep * 0.5f);
-    if (act) {
-      ke[0] += (double)vx * (double)vx;
-      ke[1] += (double)vy * (double)vy;
-      ke[2] += (double)vz * (double)vz;
+      if (ix >= g.nx || iy >= g.ny || iz >= g.nz) {  // never for a stored particle
+        if (act) atomicOr(a.flag, kFlagDomain);
+        ix = min(ix, g.nx - 1);
+        iy = min(iy, g.ny - 1);
+        iz = min(iz, g.nz - 1);
+      }
+      if (PUSH) gather_global(a.node32, g, ix, iy, iz, fx, fy, fz, f);
     }
   }
-  move_axis(fx, ix, vx, a.gstep, g.nx, a.flag, act);
-  move_axis(fy, iy, vy, a.gstep, g.ny, a.flag, act);
-  move_axis(fz, iz, vz, a.gstep, g.nz, a.flag, act);
+  if (PUSH) {
+    boris32(vx, vy, vz, f, a.qm[s] * a.gstep * 0.5f);
+    float ax = act ? vx : 0.f, ay = act ? vy : 0.f, az = act ? vz : 0.f;
+    ke[0] = fmaf(ax, ax, ke[0]);
+    ke[1] = fmaf(ay, ay, ke[1]);
+    ke[2] = fmaf(az, az, ke[2]);
+  }
+  int dx = move_offset(fx, vx, a.gstep);
+  int dy = move_offset(fy, vy, a.gstep);
+  int dz = move_offset(fz, vz, a.gstep);
+  bool far = far_move(dx, g.nx) || far_move(dy, g.ny) || far_move(dz, g.nz);
+  if (__ballot(far)) {
+    if (far) {  // also a non-finite velocity; the indices stay in range
+      if (act) atomicOr(a.flag, kFlagFar);
+      dx = max(-g.nx, min(dx, g.nx));
+      dy = max(-g.ny, min(dy, g.ny));
+      dz = max(-g.nz, min(dz, g.nz));
+    }
+  }
+  // charge tile coordinates: node (rx, ry, rz) of the (T+3)^3 tile is cell
+  // (ox - 1 + rx, ...); the particle stays in its tile iff all are in [1, T]
+  int rx = lx + dx + 1, ry = ly + dy + 1, rz = lz + dz + 1;
+  unsigned rmax = max((unsigned)rx, max((unsigned)ry, (unsigned)rz));
+  int rmin = min(rx, min(ry, rz));
+  bool stay = act && in_tile && rmin >= 1 && rmax <= (unsigned)T;
+  ix = wrap_add(ix, dx, g.nx);
+  iy = wrap_add(iy, dy, g.ny);
+  iz = wrap_add(iz, dz, g.nz);
+  // slot of a stayer: the next one of the bucket's first run
+  const int lane = __lane_id();
+  uint64_t m = __ballot(stay);
+  int sbase = 0;
+  const int sleader = __ffsll((unsigned long long)m) - 1;
+  if (m != 0 && lane == sleader) sbase = atomicAdd(n_stay, __popcll(m));
   if (DEPOSIT) {
     int W[8];
     fixed_weights(fx, fy, fz, W);
-    int rx = wrapc(ix - ox + 1, g.nx), ry = wrapc(iy - oy + 1, g.ny), rz = wrapc(iz - oz + 1, g.nz);
-    bool in_halo = lds_ok && rx <= T + 1 && ry <= T + 1 && rz <= T + 1;
+    bool in_halo = in_tile && rmax <= (unsigned)(T + 1);
     if (act && in_halo) {
       int q0 = (rz * N3 + ry) * N3 + rx;
 #pragma unroll
-      for (int q = 0; q < 8; q++) {
+      for (int q = 0; q < 7; q++) {
         int a2 = q & 1, b2 = (q >> 1) & 1, c2 = q >> 2;
-        atomicAdd(&rh[q0 + (c2 * N3 + b2) * N3 + a2], (unsigned long long)(long long)W[q]);
+        atomicAdd(&rh[q0 + (c2 * N3 + b2) * N3 + a2], (unsigned long long)(unsigned)W[q]);
       }
+      atomicAdd(&rh[q0 + (N3 + 1) * N3 + 1], (unsigned long long)(long long)W[7]);
     }
     if (__ballot(act && !in_halo)) {
       if (act && !in_halo) {
@@ -324,52 +435,44 @@ __device__ __forceinline__ void fx_one(const FxArgs &a, int s, int b_own, bool a
       }
     }
   }
-  // ---- re-bucketing
-  int dest = s * a.ntiles + ((iz / T) * a.nty + (iy / T)) * a.ntx + (ix / T);
-  bool stay = act && dest == b_own;
-  int lane = threadIdx.x & 63;
-  uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  int64_t dst = a.trash;
-  uint64_t m = __ballot(stay);
-  if (m) {
-    int leader = __ffsll((unsigned long long)m) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(n_stay, __popcll(m));
-    base = __shfl(base, leader);
-    if (stay) dst = own_end - 1 - (base + __popcll(m & lt));
+  const int cell = pack_cell(ix, iy, iz);
+  uint32_t off = kNoStore;
+  if (m != 0) {
+    sbase = __builtin_amdgcn_readlane(sbase, sleader);
+    int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                         __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+    if (stay) off = (stay_base + (uint32_t)(sbase + rank)) << 2;
   }
-  int cell = pack_cell(ix, iy, iz);
-  uint64_t mv = __ballot(act && !stay);
-  if (mv) {
-    int leader = __ffsll((unsigned long long)mv) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(n_out, __popcll(mv));
-    base = __shfl(base, leader);
-    int k = base + __popcll(mv & lt);
-    bool boxed = act && !stay && k < kOutbox;
+  const bool mover = act && !stay;
+  uint64_t mv = __ballot(mover);
+  if (mv != 0) {
+    const int leader = __ffsll((unsigned long long)mv) - 1;
+    int obase = 0;
+    if (lane == leader) obase = atomicAdd(n_out, __popcll(mv));
+    obase = __builtin_amdgcn_readlane(obase, leader);
+    int k = obase + __builtin_amdgcn_mbcnt_hi((uint32_t)(mv >> 32),
+                                              __builtin_amdgcn_mbcnt_lo((uint32_t)mv, 0));
+    const bool boxed = mover && lds_ok && k < kOutbox;
     if (boxed) {
-      OutRec r;
-      r.dest = dest;
+      FxRec r;
       r.cell = cell;
-      r.pid = p.pid;
-      r.code = 0;
-      r.rank = 0;
       r.fx = fx;
       r.fy = fy;
       r.fz = fz;
       r.vx = vx;
       r.vy = vy;
       r.vz = vz;
+      r.pid = p.pid;
       outbox[k] = r;
     }
-    if (__ballot(act && !stay && !boxed)) {  // outbox full: place directly
-      if (act && !stay && !boxed) {
-        int64_t d2 = arrival_slot(a, dest, atomicAdd(&a.back[dest], 1));
-        if (d2 >= 0) store_pcl(a.out, d2, cell, p.pid, fx, fy, fz, vx, vy, vz);
+    if (__ballot(mover && !boxed)) {  // no outbox (spill rounds) or outbox full
+      if (mover && !boxed) {
+        int dest = s * a.ntiles + tile_of_cell<T>(cell, a.ntx, a.nty);
+        place_direct(a, dest, cell, p.pid, fx, fy, fz, vx, vy, vz);
       }
     }
   }
-  store_pcl(a.out, dst, cell, p.pid, fx, fy, fz, vx, vy, vz);
+  store_masked<PID>(a.out, a.nrec, off, cell, p.pid, fx, fy, fz, vx, vy, vz);
 }
 
 __device__ __forceinline__ void block_reduce3(double ke[3], double *out) {
@@ -398,89 +501,162 @@ __device__ __forceinline__ int bucket_of_block(int bid, int nb) {
   return (bid & 7) * (nb >> 3) + (bid >> 3);
 }
 
-template <int T, bool PUSH, bool DEPOSIT>
+// Arrivals of bucket b: segment c (direction c as seen from the source) of the
+// outbox of the neighbour in direction 26 - c.  Wave 0 reads the 26 segment
+// bounds and writes their start record and the inclusive prefix of their
+// lengths (seg_pre[0] = 0, seg_pre[27] = number of arrivals).
+__device__ __forceinline__ void pull_table(const FxArgs &a, int b, int64_t *seg_src,
+                                           int *seg_pre) {
+  if (threadIdx.x < 64) {
+    const int c = threadIdx.x;
+    int n_c = 0;
+    int64_t src = 0;
+    if (c < kCodes && c != 13 && a.near_ok) {
+      int nbk = neighbour_bucket(b, 26 - c, a.ntx, a.nty, a.ntz, a.ntiles);
+      int lo = a.pre_in[(int64_t)nbk * kPre + c], hi = a.pre_in[(int64_t)nbk * kPre + c + 1];
+      n_c = hi - lo;
+      src = (int64_t)nbk * kOutbox + lo;
+    }
+    int v = n_c;
+#pragma unroll
+    for (int o = 1; o < 32; o <<= 1) {
+      int u = __shfl_up(v, o);
+      if (c >= o) v += u;
+    }
+    if (c < kCodes) {
+      seg_src[c] = src;
+      seg_pre[c + 1] = v;
+    }
+    if (c == 0) seg_pre[0] = 0;
+  }
+}
+
+// record index of arrival j: the segment c with seg_pre[c] <= j < seg_pre[c + 1]
+__device__ __forceinline__ int64_t arrival_rec(const int64_t *seg_src, const int *seg_pre, int j) {
+  int lo = 0, hi = kCodes;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (seg_pre[mid] <= j) lo = mid;
+    else hi = mid;
+  }
+  return seg_src[lo] + (j - seg_pre[lo]);
+}
+
+template <int T, bool PUSH, bool DEPOSIT, bool PID>
 __global__ __launch_bounds__(kBlock) void k_fx_step(FxArgs a) {
   constexpr int N1 = T + 1, N3 = T + 3;
+  constexpr int kPer = kOutbox / kBlock;
   __shared__ float4 fA[PUSH ? N1 * N1 * N1 : 1];
   __shared__ float2 fB[PUSH ? N1 * N1 * N1 : 1];
   __shared__ unsigned long long rh[DEPOSIT ? N3 * N3 * N3 : 1];
-  __shared__ OutRec outbox[kOutbox];
+  __shared__ FxRec outbox[kOutbox];
+  __shared__ int64_t seg_src[kCodes];
+  __shared__ int seg_pre[kPre];
   __shared__ int n_stay, n_out;
-  __shared__ int hist[kFar + 1], runbase[kFar];
-  __shared__ int64_t runstart[kFar];
-  __shared__ int runroom[kFar];
-  double ke[3] = {0.0, 0.0, 0.0};
+  __shared__ int hist[kPre], opre[kPre];
+  float ke[3] = {0.f, 0.f, 0.f};  // per lane: at most a few hundred terms
   const int bid = blockIdx.x;
   int part = bid;  // energy partial slot: bucket index, then spill blocks
-  int s = 0, b = -1, ox = 0, oy = 0, oz = 0;
+  int s = 0, b = -1;
   if (threadIdx.x == 0) {
     n_stay = 0;
     n_out = 0;
   }
-  if (threadIdx.x <= kFar) hist[threadIdx.x] = 0;
+  if (threadIdx.x < kPre) hist[threadIdx.x] = 0;
   if (bid < a.nb) {
     b = bucket_of_block(bid, a.nb);
     part = b;
     s = b / a.ntiles;
     const int t = b - s * a.ntiles;
-    ox = (t % a.ntx) * T;
-    oy = ((t / a.ntx) % a.nty) * T;
-    oz = (t / (a.ntx * a.nty)) * T;
+    const int ox = (t % a.ntx) * T, oy = ((t / a.ntx) % a.nty) * T, oz = (t / (a.ntx * a.nty)) * T;
     const int cnt = a.count_in[b];
-    const int narr = a.narr_in[b];
-    const int64_t beg = a.start_in[b];
-    const int64_t gap = a.cap_in[b] - cnt;
+    const int nst = a.nst_in[b];
+    const uint32_t beg32 = (uint32_t)a.start_in[b];
+    const uint32_t gap32 = (uint32_t)(a.cap_in[b] - cnt);
+    const uint32_t stay_base = (uint32_t)a.start_out[b];
+    auto off_of = [&](int i) -> uint32_t {
+      return (beg32 + (uint32_t)i + (i >= nst ? gap32 : 0u)) << 2;
+    };
+    auto ld = [&](int i, Pcl &p) {
+      load_off<PID>(a.in, off_of(min(i + (int)threadIdx.x, cnt - 1)), p);
+    };
+    auto one = [&](bool act, const Pcl &p) {
+      fx_one<T, PUSH, DEPOSIT, PID>(a, s, act, p, ox, oy, oz, true, stay_base, &n_stay, &n_out,
+                                    outbox, fA, fB, rh, ke);
+    };
+    // Software pipelined three deep: while a lane pushes particle k, the loads
+    // of particles k + 1 and k + 2 (the next rounds of the bucket) are in
+    // flight.  Three state sets rotate through the unrolled body, so no set is
+    // copied while its loads are outstanding.  Load indices are clamped to the
+    // bucket, so every lane loads.
+    Pcl A, B, C;
     if (cnt > 0) {
-      if (PUSH) {
-        for (int q = threadIdx.x; q < 2 * N1 * N1 * N1; q += kBlock) {
-          int nq = q >> 1;
-          int lx = nq % N1, ly = (nq / N1) % N1, lz = nq / (N1 * N1);
-          int gx = wrapc(ox + lx, a.g.nx), gy = wrapc(oy + ly, a.g.ny), gz = wrapc(oz + lz, a.g.nz);
-          const float4 *pp = reinterpret_cast<const float4 *>(
-              a.node32 + 8 * (((int64_t)gz * a.g.ny + gy) * a.g.nx + gx));
-          if (q & 1) {
-            float4 B = pp[1];
-            fB[nq] = make_float2(B.x, B.y);
-          } else {
-            fA[nq] = pp[0];
-          }
+      ld(0, A);
+      ld(kBlock, B);
+    }
+    pull_table(a, b, seg_src, seg_pre);
+    if (PUSH) {
+      for (int q = threadIdx.x; q < 2 * N1 * N1 * N1; q += kBlock) {
+        int nq = q >> 1;
+        int lx = nq % N1, ly = (nq / N1) % N1, lz = nq / (N1 * N1);
+        int gx = wrapc(ox + lx, a.g.nx), gy = wrapc(oy + ly, a.g.ny), gz = wrapc(oz + lz, a.g.nz);
+        const float4 *pp = reinterpret_cast<const float4 *>(
+            a.node32 + 8 * (((int64_t)gz * a.g.ny + gy) * a.g.nx + gx));
+        if (q & 1) {
+          float4 Bv = pp[1];
+          fB[nq] = make_float2(Bv.x, Bv.y);
+        } else {
+          fA[nq] = pp[0];
         }
       }
-      if (DEPOSIT)
-        for (int q = threadIdx.x; q < N3 * N3 * N3; q += kBlock) rh[q] = 0ull;
+    }
+    if (DEPOSIT)
+      for (int q = threadIdx.x; q < N3 * N3 * N3; q += kBlock) rh[q] = 0ull;
+    __syncthreads();
+    if (cnt > 0) {
+      const int nfull = (cnt / kBlock) * kBlock;
+      int base = 0;
+      for (; base + 3 * kBlock <= nfull; base += 3 * kBlock) {
+        ld(base + 2 * kBlock, C);
+        one(true, A);
+        ld(base + 3 * kBlock, A);
+        one(true, B);
+        ld(base + 4 * kBlock, B);
+        one(true, C);
+      }
+      // the last rounds (fewer than three full ones, then the partial one)
+      for (; base < cnt; base += kBlock) {
+        one(base + (int)threadIdx.x < cnt, A);
+        A = B;
+        ld(base + 2 * kBlock, B);
+      }
+    }
+    // arrivals pulled from the neighbours' outboxes
+    const int narr = seg_pre[kCodes];
+    for (int j0 = 0; j0 < narr; j0 += kBlock) {
+      const int j = j0 + (int)threadIdx.x;
+      Pcl P;
+      load_rec(a.ob_in, arrival_rec(seg_src, seg_pre, min(j, narr - 1)), P);
+      if (!PID) P.pid = 0;
+      one(j < narr, P);
+    }
+    if (DEPOSIT) {
       __syncthreads();
-      const int64_t own_end = a.start_out[b] + a.cap_out[b];
-      // software pipelined: the next particle's state is in flight while the
-      // current one is pushed (the load index is clamped, so every lane loads)
-      Pcl cur, nxt;
-      {
-        int i = min((int)threadIdx.x, cnt - 1);
-        load_pcl(a.in, beg + i + (i >= narr ? gap : 0), cur);
-      }
-      for (int base = 0; base < cnt; base += kBlock) {
-        int i = min(base + kBlock + (int)threadIdx.x, cnt - 1);
-        load_pcl(a.in, beg + i + (i >= narr ? gap : 0), nxt);
-        fx_one<T, PUSH, DEPOSIT>(a, s, b, base + (int)threadIdx.x < cnt, cur, ox, oy, oz, true,
-                                 own_end, &n_stay, &n_out, outbox, fA, fB, rh, ke);
-        cur = nxt;
-      }
-      if (DEPOSIT) {
-        __syncthreads();
-        unsigned long long *r = a.rho + (int64_t)s * a.ncell;
-        for (int q = threadIdx.x; q < N3 * N3 * N3; q += kBlock) {
-          unsigned long long v = rh[q];
-          if (v != 0ull) {
-            int rx = q % N3, ry = (q / N3) % N3, rz = q / (N3 * N3);
-            int gx = wrapc(ox - 1 + rx, a.g.nx);
-            int gy = wrapc(oy - 1 + ry, a.g.ny);
-            int gz = wrapc(oz - 1 + rz, a.g.nz);
-            atomicAdd(&r[((int64_t)gz * a.g.ny + gy) * a.g.nx + gx], v);
-          }
+      unsigned long long *r = a.rho + (int64_t)s * a.ncell;
+      for (int q = threadIdx.x; q < N3 * N3 * N3; q += kBlock) {
+        unsigned long long v = rh[q];
+        if (v != 0ull) {
+          int rx = q % N3, ry = (q / N3) % N3, rz = q / (N3 * N3);
+          int gx = wrapc(ox - 1 + rx, a.g.nx);
+          int gy = wrapc(oy - 1 + ry, a.g.ny);
+          int gz = wrapc(oz - 1 + rz, a.g.nz);
+          atomicAdd(&r[((int64_t)gz * a.g.ny + gy) * a.g.nx + gx], v);
         }
       }
     }
   } else {
-    // spill particles of the previous step: global-memory gather and deposit
+    // spill particles of the previous step: global-memory gather and deposit,
+    // every particle placed directly
     int k = bid - a.nb;
     s = k / kSpillBlocksPerSpecies;
     int sub = k % kSpillBlocksPerSpecies;
@@ -493,66 +669,107 @@ __global__ __launch_bounds__(kBlock) void k_fx_step(FxArgs a) {
       int i = (r * kSpillBlocksPerSpecies + sub) * kBlock + threadIdx.x;
       Pcl cur{};
       if (i < cnt) load_pcl(a.in, beg + i, cur);
-      fx_one<T, PUSH, DEPOSIT>(a, s, -1, i < cnt, cur, 0, 0, 0, false, 0, &n_stay, &n_out, outbox,
-                               fA, fB, rh, ke);
+      fx_one<T, PUSH, DEPOSIT, PID>(a, s, i < cnt, cur, 0, 0, 0, false, 0u, &n_stay, &n_out,
+                                    outbox, fA, fB, rh, ke);
     }
   }
-  // ---- place the outbox: one reserved run per destination bucket
+  // ---- write the outbox grouped by direction (bucket blocks)
   __syncthreads();
-  const int nout = min(n_out, kOutbox);
-  for (int q = threadIdx.x; q < nout; q += kBlock) {
-    int c = dest_code(outbox[q].dest, b, a.ntx, a.nty, a.ntz, a.ntiles);
-    outbox[q].code = c;
-    if (c < kFar) outbox[q].rank = atomicAdd(&hist[c], 1);
-  }
-  __syncthreads();
-  if (threadIdx.x < kFar && hist[threadIdx.x] > 0) {
-    int d = neighbour_bucket(b, threadIdx.x, a.ntx, a.nty, a.ntz, a.ntiles);
-    runbase[threadIdx.x] = atomicAdd(&a.back[d], hist[threadIdx.x]);
-    runstart[threadIdx.x] = a.start_out[d];
-    runroom[threadIdx.x] = a.cap_out[d] - a.count_in[d];
-  }
-  if (threadIdx.x == 0 && b >= 0) a.front[b] = n_stay;
-  __syncthreads();
-  for (int q = threadIdx.x; q < nout; q += kBlock) {
-    OutRec r = outbox[q];
-    int64_t dst;
-    if (r.code < kFar) {
-      int pos = runbase[r.code] + r.rank;
-      dst = pos < runroom[r.code] ? runstart[r.code] + pos : arrival_slot(a, r.dest, pos);
-    } else {
-      dst = arrival_slot(a, r.dest, atomicAdd(&a.back[r.dest], 1));
+  if (b >= 0) {
+    const int nout = min(n_out, kOutbox);
+    const int t = b - s * a.ntiles;
+    int cq[kPer], rq[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      const int q = (int)threadIdx.x + k * kBlock;
+      cq[k] = kFar;
+      rq[k] = 0;
+      if (q < nout) {
+        cq[k] = dir_code(tile_of_cell<T>(outbox[q].cell, a.ntx, a.nty), t, a.ntx, a.nty, a.ntz,
+                         a.near_ok);
+        if (cq[k] < kFar) rq[k] = atomicAdd(&hist[cq[k]], 1);
+      }
     }
-    if (dst >= 0) store_pcl(a.out, dst, r.cell, r.pid, r.fx, r.fy, r.fz, r.vx, r.vy, r.vz);
+    __syncthreads();
+    if (threadIdx.x < 64) {
+      const int c = threadIdx.x;
+      int v = c < kCodes ? hist[c] : 0;
+      int incl = v;
+#pragma unroll
+      for (int o = 1; o < 32; o <<= 1) {
+        int u = __shfl_up(incl, o);
+        if (c >= o) incl += u;
+      }
+      if (c < kPre) {
+        int ex = incl - v;  // c = 27: the total
+        opre[c] = ex;
+        a.pre_out[(int64_t)b * kPre + c] = ex;
+      }
+    }
+    if (threadIdx.x == 0) a.stay_out[b] = n_stay;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      const int q = (int)threadIdx.x + k * kBlock;
+      if (q < nout) {
+        const FxRec r = outbox[q];
+        if (cq[k] < kFar) {
+          a.ob_out[(int64_t)b * kOutbox + opre[cq[k]] + rq[k]] = r;
+        } else {
+          place_direct(a, s * a.ntiles + tile_of_cell<T>(r.cell, a.ntx, a.nty), r.cell, r.pid,
+                       r.fx, r.fy, r.fz, r.vx, r.vy, r.vz);
+        }
+      }
+    }
   }
-  if (PUSH) block_reduce3(ke, a.ke_part + 3 * (int64_t)part);
+  if (PUSH) {
+    double kd[3] = {(double)ke[0], (double)ke[1], (double)ke[2]};
+    block_reduce3(kd, a.ke_part + 3 * (int64_t)part);
+  }
 }
 
-// standalone charge deposition of the current positions (no move, no bucket change)
+// standalone charge deposition of the current state (no move, no bucket change):
+// each bucket's runs plus the arrivals waiting in its neighbours' outboxes
 template <int T>
 __global__ __launch_bounds__(kBlock) void k_fx_deposit(FxArgs a) {
   constexpr int N3 = T + 3;
   __shared__ unsigned long long rh[N3 * N3 * N3];
+  __shared__ int64_t seg_src[kCodes];
+  __shared__ int seg_pre[kPre];
   const int bid = blockIdx.x;
   const Grid &g = a.g;
+  auto deposit_global = [&](unsigned long long *r, int ix, int iy, int iz, const int W[8]) {
+    int xs[2] = {ix, ix + 1 == g.nx ? 0 : ix + 1};
+    int ys[2] = {iy, iy + 1 == g.ny ? 0 : iy + 1};
+    int zs[2] = {iz, iz + 1 == g.nz ? 0 : iz + 1};
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      int a2 = q & 1, b2 = (q >> 1) & 1, c2 = q >> 2;
+      atomicAdd(&r[((int64_t)zs[c2] * g.ny + ys[b2]) * g.nx + xs[a2]],
+                (unsigned long long)(long long)W[q]);
+    }
+  };
   if (bid < a.nb) {
     const int b = bucket_of_block(bid, a.nb);
     const int s = b / a.ntiles, t = b - s * a.ntiles;
     const int ox = (t % a.ntx) * T, oy = ((t / a.ntx) % a.nty) * T, oz = (t / (a.ntx * a.nty)) * T;
     const int cnt = a.count_in[b];
-    const int narr = a.narr_in[b];
+    const int nst = a.nst_in[b];
     const int64_t beg = a.start_in[b];
     const int64_t gap = a.cap_in[b] - cnt;
-    if (cnt == 0) return;
+    pull_table(a, b, seg_src, seg_pre);
     for (int q = threadIdx.x; q < N3 * N3 * N3; q += kBlock) rh[q] = 0ull;
     __syncthreads();
+    const int narr = seg_pre[kCodes];
     unsigned long long *r = a.rho + (int64_t)s * a.ncell;
-    for (int i = threadIdx.x; i < cnt; i += kBlock) {
-      int64_t src = beg + i + (i >= narr ? gap : 0);
-      int cell = a.in.cell[src];
-      int ix = cell & kCellMask, iy = (cell >> kCellBits) & kCellMask, iz = cell >> (2 * kCellBits);
+    for (int i = threadIdx.x; i < cnt + narr; i += kBlock) {
+      Pcl p;
+      if (i < cnt) load_pcl(a.in, beg + i + (i >= nst ? gap : 0), p);
+      else load_rec(a.ob_in, arrival_rec(seg_src, seg_pre, i - cnt), p);
+      int ix = min(p.cell & kCellMask, g.nx - 1), iy = min((p.cell >> kCellBits) & kCellMask, g.ny - 1),
+          iz = min((int)((unsigned)p.cell >> (2 * kCellBits)), g.nz - 1);
       int W[8];
-      fixed_weights(a.in.fx[src], a.in.fy[src], a.in.fz[src], W);
+      fixed_weights(p.fx, p.fy, p.fz, W);
       int rx = wrapc(ix - ox + 1, g.nx), ry = wrapc(iy - oy + 1, g.ny), rz = wrapc(iz - oz + 1, g.nz);
       if (rx <= T + 1 && ry <= T + 1 && rz <= T + 1) {
 #pragma unroll
@@ -562,15 +779,7 @@ __global__ __launch_bounds__(kBlock) void k_fx_deposit(FxArgs a) {
                     (unsigned long long)(long long)W[q]);
         }
       } else {
-        int xs[2] = {ix, ix + 1 == g.nx ? 0 : ix + 1};
-        int ys[2] = {iy, iy + 1 == g.ny ? 0 : iy + 1};
-        int zs[2] = {iz, iz + 1 == g.nz ? 0 : iz + 1};
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-          int a2 = q & 1, b2 = (q >> 1) & 1, c2 = q >> 2;
-          atomicAdd(&r[((int64_t)zs[c2] * g.ny + ys[b2]) * g.nx + xs[a2]],
-                    (unsigned long long)(long long)W[q]);
-        }
+        deposit_global(r, ix, iy, iz, W);
       }
     }
     __syncthreads();
@@ -590,101 +799,153 @@ __global__ __launch_bounds__(kBlock) void k_fx_deposit(FxArgs a) {
     int64_t beg = a.area + (int64_t)s * a.spill_cap;
     unsigned long long *r = a.rho + (int64_t)s * a.ncell;
     for (int i = sub * kBlock + threadIdx.x; i < cnt; i += kBlock * kSpillBlocksPerSpecies) {
-      int64_t src = beg + i;
-      int cell = a.in.cell[src];
-      int ix = cell & kCellMask, iy = (cell >> kCellBits) & kCellMask, iz = cell >> (2 * kCellBits);
+      Pcl p;
+      load_pcl(a.in, beg + i, p);
+      int ix = min(p.cell & kCellMask, g.nx - 1), iy = min((p.cell >> kCellBits) & kCellMask, g.ny - 1),
+          iz = min((int)((unsigned)p.cell >> (2 * kCellBits)), g.nz - 1);
       int W[8];
-      fixed_weights(a.in.fx[src], a.in.fy[src], a.in.fz[src], W);
-      int xs[2] = {ix, ix + 1 == g.nx ? 0 : ix + 1};
-      int ys[2] = {iy, iy + 1 == g.ny ? 0 : iy + 1};
-      int zs[2] = {iz, iz + 1 == g.nz ? 0 : iz + 1};
-#pragma unroll
-      for (int q = 0; q < 8; q++) {
-        int a2 = q & 1, b2 = (q >> 1) & 1, c2 = q >> 2;
-        atomicAdd(&r[((int64_t)zs[c2] * g.ny + ys[b2]) * g.nx + xs[a2]],
-                  (unsigned long long)(long long)W[q]);
-      }
+      fixed_weights(p.fx, p.fy, p.fz, W);
+      deposit_global(r, ix, iy, iz, W);
     }
   }
 }
 
-// output capacities of a step from the input counts; clears the spill counters
-__global__ void k_fx_caps(const int32_t *count_in, int32_t *cap_out, int32_t *spill_out, int nb,
-                          int nspec) {
+// kinetic energy of the FP32 state (when no fused push partials exist)
+__global__ __launch_bounds__(kBlock) void k_fx_energy(FxArgs a) {
+  __shared__ int64_t seg_src[kCodes];
+  __shared__ int seg_pre[kPre];
+  double ke[3] = {0.0, 0.0, 0.0};
+  const int bid = blockIdx.x;
+  auto add = [&](const Pcl &p) {
+    ke[0] += (double)p.vx * p.vx;
+    ke[1] += (double)p.vy * p.vy;
+    ke[2] += (double)p.vz * p.vz;
+  };
+  if (bid < a.nb) {
+    const int cnt = a.count_in[bid], nst = a.nst_in[bid];
+    const int64_t beg = a.start_in[bid], gap = a.cap_in[bid] - cnt;
+    pull_table(a, bid, seg_src, seg_pre);
+    __syncthreads();
+    const int narr = seg_pre[kCodes];
+    for (int i = threadIdx.x; i < cnt + narr; i += kBlock) {
+      Pcl p;
+      if (i < cnt) load_pcl(a.in, beg + i + (i >= nst ? gap : 0), p);
+      else load_rec(a.ob_in, arrival_rec(seg_src, seg_pre, i - cnt), p);
+      add(p);
+    }
+  } else {
+    const int s = bid - a.nb;
+    const int cnt = min(a.spill_in[s], (int32_t)a.spill_cap);
+    const int64_t beg = a.area + (int64_t)s * a.spill_cap;
+    for (int i = threadIdx.x; i < cnt; i += kBlock) {
+      Pcl p;
+      load_pcl(a.in, beg + i, p);
+      add(p);
+    }
+  }
+  block_reduce3(ke, a.ke_part + 3 * (int64_t)bid);
+}
+
+// output capacities of a step: the bucket's particles, its arrivals waiting in
+// the neighbours' outboxes and kExtra direct slots; clears the step counters
+__global__ void k_fx_caps(const int32_t *count_in, const int32_t *pre_in, int32_t *cap_out,
+                          int32_t *extra, int32_t *spill_out, int nb, int nspec, int ntx, int nty,
+                          int ntz, int ntiles, int near_ok) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b < nb) {
-    int c = count_in[b];
-    cap_out[b] = c + c / 4 + 64;
+    int arr = 0;
+    if (near_ok)
+      for (int c = 0; c < kCodes; c++) {
+        if (c == 13) continue;
+        int n = neighbour_bucket(b, 26 - c, ntx, nty, ntz, ntiles);
+        arr += pre_in[(int64_t)n * kPre + c + 1] - pre_in[(int64_t)n * kPre + c];
+      }
+    cap_out[b] = count_in[b] + arr + kExtra;
+    extra[b] = 0;
   }
   if (b < nspec) spill_out[b] = 0;
 }
 
-__global__ void k_fill_i32(int32_t *p, int n, int v) {
+// capacities of freshly bucketed particles (no outbox contents)
+__global__ void k_fx_caps_hist(const int32_t *hist, int32_t *cap_out, int32_t *spill_out, int nb,
+                               int nspec) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < nb) cap_out[b] = hist[b] + kExtra;
+  if (b < nspec) spill_out[b] = 0;
+}
+
+__global__ void k_fill_i32(int32_t *p, int64_t n, int v) {
+  int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b < n) p[b] = v;
 }
 
-// after a step: the output buckets' run lengths (arrivals that did not fit
-// were sent to the spill region), spill counters clamped
-__global__ void k_fx_finish(const int32_t *front, const int32_t *back, const int32_t *count_in,
-                            const int32_t *cap_out, int32_t *narr_out, int32_t *count_out, int nb,
-                            int32_t *spill, int nspec, int32_t spill_cap) {
+// after a step: the output buckets' runs (direct arrivals beyond kExtra went
+// to the spill region), spill counters clamped
+__global__ void k_fx_finish(const int32_t *stay, const int32_t *extra, int32_t *nst_out,
+                            int32_t *count_out, int nb, int32_t *spill, int nspec,
+                            int32_t spill_cap) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b < nb) {
-    int na = min(back[b], cap_out[b] - count_in[b]);
-    narr_out[b] = na;
-    count_out[b] = na + front[b];
+    int ne = min(extra[b], kExtra);
+    nst_out[b] = stay[b];
+    count_out[b] = stay[b] + ne;
   }
   if (b < nspec) spill[b] = min(spill[b], spill_cap);
 }
 
-// ---- bucketing of input ranges (flat species segments, spill regions or
-// existing buckets) into the buckets of the other buffer
+// ---- bucketing of input ranges (existing buckets, flat species segments or
+// spill regions, and outboxes) into the buckets of the other buffer
 struct RangeDesc {
   int nrb;     // ranges [0, nrb) are buckets (species r / ntiles)
   int nspec;   // ranges [nrb, nrb + nspec) are per-species segments
+  int nob;     // ranges [nrb + nspec, nrb + nspec + nob) are bucket outboxes
   int ntiles;
   int rb;      // blocks per range
   const int64_t *bstart;
-  const int32_t *bcount, *bnarr, *bcap;
+  const int32_t *bcount, *bnst, *bcap;
   int64_t seg_start[EMG_MAX_SPECIES];
   const int32_t *seg_count_dev;        // device counts (spill) or nullptr
   int64_t seg_count[EMG_MAX_SPECIES];  // host counts (flat)
+  const int32_t *pre;
+  const FxRec *ob;
 };
 
-__device__ __forceinline__ void range_of(const RangeDesc &d, int r, int64_t &beg, int64_t &cnt,
-                                         int64_t &narr, int64_t &gap, int &s) {
+__device__ __forceinline__ int64_t range_count(const RangeDesc &d, int r, int &s) {
   if (r < d.nrb) {
-    beg = d.bstart[r];
-    cnt = d.bcount[r];
-    narr = d.bnarr[r];
-    gap = d.bcap[r] - cnt;
     s = r / d.ntiles;
-  } else {
-    s = r - d.nrb;
-    beg = d.seg_start[s];
-    cnt = d.seg_count_dev ? (int64_t)d.seg_count_dev[s] : d.seg_count[s];
-    narr = cnt;
-    gap = 0;
+    return d.bcount[r];
   }
+  if (r < d.nrb + d.nspec) {
+    s = r - d.nrb;
+    return d.seg_count_dev ? (int64_t)d.seg_count_dev[s] : d.seg_count[s];
+  }
+  int n = r - d.nrb - d.nspec;
+  s = n / d.ntiles;
+  return d.pre[(int64_t)n * kPre + kCodes];
 }
 
-template <int T>
-__device__ __forceinline__ int bucket_of_cell(int cell, int s, int ntx, int nty, int ntiles) {
-  int ix = cell & kCellMask, iy = (cell >> kCellBits) & kCellMask, iz = cell >> (2 * kCellBits);
-  return s * ntiles + ((iz / T) * nty + (iy / T)) * ntx + (ix / T);
+__device__ __forceinline__ void range_load(const RangeDesc &d, const StateFx &in, int r,
+                                           int64_t i, Pcl &p) {
+  if (r < d.nrb) {
+    int64_t cnt = d.bcount[r];
+    load_pcl(in, d.bstart[r] + i + (i >= d.bnst[r] ? d.bcap[r] - cnt : 0), p);
+  } else if (r < d.nrb + d.nspec) {
+    load_pcl(in, d.seg_start[r - d.nrb] + i, p);
+  } else {
+    load_rec(d.ob, (int64_t)(r - d.nrb - d.nspec) * kOutbox + i, p);
+  }
 }
 
 template <int T>
 __global__ __launch_bounds__(kBlock) void k_fx_hist(RangeDesc d, StateFx in, int ntx, int nty,
                                                     int32_t *hist) {
   int r = blockIdx.x / d.rb, sub = blockIdx.x % d.rb;
-  int64_t beg, cnt, narr, gap;
   int s;
-  range_of(d, r, beg, cnt, narr, gap, s);
+  int64_t cnt = range_count(d, r, s);
   for (int64_t i = (int64_t)sub * kBlock + threadIdx.x; i < cnt; i += (int64_t)d.rb * kBlock) {
-    int b = bucket_of_cell<T>(in.cell[beg + i + (i >= narr ? gap : 0)], s, ntx, nty, d.ntiles);
-    atomicAdd(&hist[b], 1);
+    Pcl p;
+    range_load(d, in, r, i, p);
+    atomicAdd(&hist[s * d.ntiles + tile_of_cell<T>(p.cell, ntx, nty)], 1);
   }
 }
 
@@ -693,16 +954,14 @@ __global__ __launch_bounds__(kBlock) void k_fx_scatter(RangeDesc d, StateFx in, 
                                                        int ntx, int nty, const int64_t *start,
                                                        int32_t *cursor) {
   int r = blockIdx.x / d.rb, sub = blockIdx.x % d.rb;
-  int64_t beg, cnt, narr, gap;
   int s;
-  range_of(d, r, beg, cnt, narr, gap, s);
+  int64_t cnt = range_count(d, r, s);
   for (int64_t i = (int64_t)sub * kBlock + threadIdx.x; i < cnt; i += (int64_t)d.rb * kBlock) {
-    int64_t src = beg + i + (i >= narr ? gap : 0);
-    int cell = in.cell[src];
-    int b = bucket_of_cell<T>(cell, s, ntx, nty, d.ntiles);
+    Pcl p;
+    range_load(d, in, r, i, p);
+    int b = s * d.ntiles + tile_of_cell<T>(p.cell, ntx, nty);
     int64_t dst = start[b] + atomicAdd(&cursor[b], 1);
-    store_pcl(out, dst, cell, in.pid ? in.pid[src] : 0, in.fx[src], in.fy[src], in.fz[src],
-              in.vx[src], in.vy[src], in.vz[src]);
+    store_pcl(out, dst, p.cell, p.pid, p.fx, p.fy, p.fz, p.vx, p.vy, p.vz);
   }
 }
 
@@ -739,18 +998,18 @@ __global__ __launch_bounds__(kBlock) void k_fx_from64(State64 s64, SpeciesTab sp
             (float)s64.vy[m], (float)s64.vz[m]);
 }
 
-// bucketed FP32 -> FP64 species-contiguous (dst offset = flat scan of counts)
+// bucketed FP32 (one run per bucket) -> FP64 species-contiguous (dst offset =
+// flat scan of counts)
 __global__ __launch_bounds__(kBlock) void k_fx_to64(StateFx in, const int64_t *bstart,
-                                                    const int32_t *bcount, const int32_t *bnarr,
-                                                    const int32_t *bcap, const int64_t *flat,
+                                                    const int32_t *bcount, const int64_t *flat,
                                                     int ntiles, SpeciesTab sp, State64 out) {
   int b = blockIdx.x;
   int s = b / ntiles;
-  int cnt = bcount[b], narr = bnarr[b];
-  int64_t beg = bstart[b], gap = bcap[b] - cnt;
+  int cnt = bcount[b];
+  int64_t beg = bstart[b];
   int64_t base = sp.off[s] + (flat[b] - flat[s * ntiles]);
   for (int i = threadIdx.x; i < cnt; i += kBlock) {
-    int64_t src = beg + i + (i >= narr ? gap : 0), dst = base + i;
+    int64_t src = beg + i, dst = base + i;
     int cell = in.cell[src];
     int ix = cell & kCellMask, iy = (cell >> kCellBits) & kCellMask, iz = cell >> (2 * kCellBits);
     out.x[dst] = (double)ix + (double)in.fx[src];
@@ -761,37 +1020,6 @@ __global__ __launch_bounds__(kBlock) void k_fx_to64(StateFx in, const int64_t *b
     out.vz[dst] = (double)in.vz[src];
     out.pid[dst] = in.pid ? in.pid[src] : 0;
   }
-}
-
-// kinetic energy of the bucketed FP32 state (when no fused push partials exist)
-__global__ __launch_bounds__(kBlock) void k_fx_energy(StateFx in, const int64_t *bstart,
-                                                      const int32_t *bcount, const int32_t *bnarr,
-                                                      const int32_t *bcap, int nb,
-                                                      const int32_t *spill, int64_t area,
-                                                      int64_t spill_cap, double *part) {
-  double ke[3] = {0.0, 0.0, 0.0};
-  int bid = blockIdx.x;
-  int64_t beg, gap = 0;
-  int cnt, narr;
-  if (bid < nb) {
-    beg = bstart[bid];
-    cnt = bcount[bid];
-    narr = bnarr[bid];
-    gap = bcap[bid] - cnt;
-  } else {
-    int s = bid - nb;
-    beg = area + (int64_t)s * spill_cap;
-    cnt = spill[s];
-    narr = cnt;
-  }
-  for (int i = threadIdx.x; i < cnt; i += kBlock) {
-    int64_t src = beg + i + (i >= narr ? gap : 0);
-    float vx = in.vx[src], vy = in.vy[src], vz = in.vz[src];
-    ke[0] += (double)vx * vx;
-    ke[1] += (double)vy * vy;
-    ke[2] += (double)vz * vz;
-  }
-  block_reduce3(ke, part + 3 * (int64_t)bid);
 }
 
 // per-species sum of block partials in a fixed order; partials are indexed by
@@ -820,6 +1048,8 @@ __global__ void k_copy_i32(const int32_t *src, int n, int32_t *dst) {
   if (b < n) dst[b] = src[b];
 }
 
+int near_ok(const FxBuckets &B) { return B.ntx >= 3 && B.nty >= 3 && B.ntz >= 3; }
+
 FxArgs make_args(emg_ctx *c) {
   FxBuckets &B = c->fx;
   int in = B.cur, out = 1 - B.cur;
@@ -831,21 +1061,26 @@ FxArgs make_args(emg_ctx *c) {
   a.ntiles = B.ntiles;
   a.nb = B.nb;
   a.nspec = c->sp.nspec;
+  a.near_ok = near_ok(B);
   a.in = B.st[in];
   a.out = B.st[out];
   a.start_in = B.start[in];
   a.count_in = B.count[in];
-  a.narr_in = B.narr[in];
+  a.nst_in = B.nst[in];
   a.cap_in = B.cap[in];
   a.spill_in = B.spill[in];
+  a.pre_in = B.pre[in];
+  a.ob_in = B.ob[in];
   a.start_out = B.start[out];
   a.cap_out = B.cap[out];
-  a.front = B.front;
-  a.back = B.back;
+  a.stay_out = B.stay;
+  a.extra = B.extra;
   a.spill_out = B.spill[out];
+  a.pre_out = B.pre[out];
+  a.ob_out = B.ob[out];
   a.area = B.area;
   a.spill_cap = B.spill_cap;
-  a.trash = B.trash;
+  a.nrec = (int32_t)(B.cap_total * 4);
   a.node32 = c->node32;
   a.rho = c->rhofx;
   a.ncell = c->ncell;
@@ -856,16 +1091,24 @@ FxArgs make_args(emg_ctx *c) {
   return a;
 }
 
+template <int T, bool PID>
+void step_tp(emg_ctx *c, const FxArgs &a, bool push, bool deposit, unsigned grid) {
+  if (push && deposit)
+    hipLaunchKernelGGL((k_fx_step<T, true, true, PID>), dim3(grid), dim3(kBlock), 0, c->stream, a);
+  else if (push)
+    hipLaunchKernelGGL((k_fx_step<T, true, false, PID>), dim3(grid), dim3(kBlock), 0, c->stream, a);
+  else if (deposit)
+    hipLaunchKernelGGL((k_fx_step<T, false, true, PID>), dim3(grid), dim3(kBlock), 0, c->stream, a);
+  else
+    hipLaunchKernelGGL((k_fx_step<T, false, false, PID>), dim3(grid), dim3(kBlock), 0, c->stream, a);
+}
+
 template <int T>
 void step_t(emg_ctx *c, const FxArgs &a, bool push, bool deposit, unsigned grid) {
-  if (push && deposit)
-    hipLaunchKernelGGL((k_fx_step<T, true, true>), dim3(grid), dim3(kBlock), 0, c->stream, a);
-  else if (push)
-    hipLaunchKernelGGL((k_fx_step<T, true, false>), dim3(grid), dim3(kBlock), 0, c->stream, a);
-  else if (deposit)
-    hipLaunchKernelGGL((k_fx_step<T, false, true>), dim3(grid), dim3(kBlock), 0, c->stream, a);
+  if (a.in.pid)
+    step_tp<T, true>(c, a, push, deposit, grid);
   else
-    hipLaunchKernelGGL((k_fx_step<T, false, false>), dim3(grid), dim3(kBlock), 0, c->stream, a);
+    step_tp<T, false>(c, a, push, deposit, grid);
 }
 
 // bucket the particles described by d (reading buffer 'src') into buffer 1 - src
@@ -888,9 +1131,9 @@ int bucket_ranges(emg_ctx *c, RangeDesc d, int nr, int rb, int src) {
     default: return set_error(EMG_ERR_ARG, "tile must be 4 or 8");
   }
   // capacities from the histogram, start offsets, cursor = 0, spill = 0
-  hipLaunchKernelGGL(k_fx_caps, dim3(gb), dim3(kBlock), 0, c->stream, B.hist, B.cap[dst],
+  hipLaunchKernelGGL(k_fx_caps_hist, dim3(gb), dim3(kBlock), 0, c->stream, B.hist, B.cap[dst],
                      B.spill[dst], B.nb, c->sp.nspec);
-  hipLaunchKernelGGL(k_fill_i32, dim3(gb), dim3(kBlock), 0, c->stream, B.cursor, B.nb, 0);
+  hipLaunchKernelGGL(k_fill_i32, dim3(gb), dim3(kBlock), 0, c->stream, B.cursor, (int64_t)B.nb, 0);
   exclusive_scan_i32(B.cap[dst], B.start[dst], B.nb, c->scan_tmp, c->stream);
   switch (B.tile) {
     case 4:
@@ -902,9 +1145,12 @@ int bucket_ranges(emg_ctx *c, RangeDesc d, int nr, int rb, int src) {
                          B.st[dst], B.ntx, B.nty, B.start[dst], B.cursor);
       break;
   }
-  // all particles of a bucket form one run from its start
+  // all particles of a bucket form one run from its start; no outbox contents
   hipLaunchKernelGGL(k_copy_i32, dim3(gb), dim3(kBlock), 0, c->stream, B.cursor, B.nb, B.count[dst]);
-  hipLaunchKernelGGL(k_copy_i32, dim3(gb), dim3(kBlock), 0, c->stream, B.cursor, B.nb, B.narr[dst]);
+  hipLaunchKernelGGL(k_copy_i32, dim3(gb), dim3(kBlock), 0, c->stream, B.cursor, B.nb, B.nst[dst]);
+  int64_t npre = (int64_t)B.nb * kPre;
+  hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((npre + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                     c->stream, B.pre[dst], npre, 0);
   B.cur = dst;
   return hip_check(hipGetLastError(), "bucketing");
 }
@@ -914,13 +1160,16 @@ RangeDesc bucket_desc(emg_ctx *c) {
   RangeDesc d{};
   d.nrb = B.nb;
   d.nspec = c->sp.nspec;
+  d.nob = B.nb;
   d.ntiles = B.ntiles;
   d.bstart = B.start[B.cur];
   d.bcount = B.count[B.cur];
-  d.bnarr = B.narr[B.cur];
+  d.bnst = B.nst[B.cur];
   d.bcap = B.cap[B.cur];
   d.seg_count_dev = B.spill[B.cur];
   for (int s = 0; s < c->sp.nspec; s++) d.seg_start[s] = B.area + (int64_t)s * B.spill_cap;
+  d.pre = B.pre[B.cur];
+  d.ob = B.ob[B.cur];
   return d;
 }
 
@@ -940,13 +1189,17 @@ int fx_alloc(emg_ctx *c, int64_t ntotal) {
   B.ntz = c->g.nz / T;
   B.ntiles = B.ntx * B.nty * B.ntz;
   B.nb = B.ntiles * c->sp.nspec;
-  B.area = ntotal + ntotal / 4 + 64 * (int64_t)B.nb;
+  // every particle is in a bucket run, an outbox or the spill region, so the
+  // buckets' capacities (count + arrivals + kExtra) sum to at most this
+  B.area = ntotal + (int64_t)kExtra * B.nb;
   int64_t maxs = 0;
   for (int s = 0; s < c->sp.nspec; s++) maxs = c->cfg.capacity[s] > maxs ? c->cfg.capacity[s] : maxs;
   B.spill_cap = maxs / 64 + 4096;
-  if (B.spill_cap > (1 << 30)) B.spill_cap = 1 << 30;
-  B.cap_total = B.area + B.spill_cap * c->sp.nspec + 1;  // + the trash slot
-  B.trash = B.cap_total - 1;
+  if (B.spill_cap > (1 << 28)) B.spill_cap = 1 << 28;
+  B.cap_total = B.area + B.spill_cap * c->sp.nspec;
+  if (B.cap_total >= (int64_t(1) << 29))  // masked stores: byte offsets below 2^31
+    return set_error(EMG_ERR_ARG, "FP32_FIXED holds at most 2^29 particle slots per context "
+                                  "(about 500M particles)");
   for (int k = 0; k < 2; k++) {
     StateFx &S = B.st[k];
     void *p[7];
@@ -966,18 +1219,22 @@ int fx_alloc(emg_ctx *c, int64_t ntotal) {
       return set_error(EMG_ERR_NOMEM, "FP32 pid buffers");
     if (hipMalloc(&B.start[k], sizeof(int64_t) * (B.nb + 1)) != hipSuccess ||
         hipMalloc(&B.count[k], sizeof(int32_t) * B.nb) != hipSuccess ||
-        hipMalloc(&B.narr[k], sizeof(int32_t) * B.nb) != hipSuccess ||
+        hipMalloc(&B.nst[k], sizeof(int32_t) * B.nb) != hipSuccess ||
         hipMalloc(&B.cap[k], sizeof(int32_t) * B.nb) != hipSuccess ||
-        hipMalloc(&B.spill[k], sizeof(int32_t) * EMG_MAX_SPECIES) != hipSuccess)
+        hipMalloc(&B.spill[k], sizeof(int32_t) * EMG_MAX_SPECIES) != hipSuccess ||
+        hipMalloc(&B.pre[k], sizeof(int32_t) * B.nb * kPre) != hipSuccess)
       return set_error(EMG_ERR_NOMEM, "bucket tables");
+    if (hipMalloc(&B.ob[k], sizeof(FxRec) * B.nb * kOutbox) != hipSuccess)
+      return set_error(EMG_ERR_NOMEM, "bucket outboxes");
     launch_zero(B.count[k], sizeof(int32_t) * B.nb, c->stream);
-    launch_zero(B.narr[k], sizeof(int32_t) * B.nb, c->stream);
+    launch_zero(B.nst[k], sizeof(int32_t) * B.nb, c->stream);
     launch_zero(B.cap[k], sizeof(int32_t) * B.nb, c->stream);
     launch_zero(B.spill[k], sizeof(int32_t) * EMG_MAX_SPECIES, c->stream);
     launch_zero(B.start[k], sizeof(int64_t) * (B.nb + 1), c->stream);
+    launch_zero(B.pre[k], sizeof(int32_t) * B.nb * kPre, c->stream);
   }
-  if (hipMalloc(&B.front, sizeof(int32_t) * B.nb) != hipSuccess ||
-      hipMalloc(&B.back, sizeof(int32_t) * B.nb) != hipSuccess ||
+  if (hipMalloc(&B.stay, sizeof(int32_t) * B.nb) != hipSuccess ||
+      hipMalloc(&B.extra, sizeof(int32_t) * B.nb) != hipSuccess ||
       hipMalloc(&B.hist, sizeof(int32_t) * B.nb) != hipSuccess ||
       hipMalloc(&B.cursor, sizeof(int32_t) * B.nb) != hipSuccess)
     return set_error(EMG_ERR_NOMEM, "bucket tables");
@@ -1007,6 +1264,7 @@ int fx_from64(emg_ctx *c, const State64 &src) {
   RangeDesc d{};
   d.nrb = 0;
   d.nspec = c->sp.nspec;
+  d.nob = 0;
   d.ntiles = B.ntiles;
   for (int s = 0; s < c->sp.nspec; s++) {
     d.seg_start[s] = c->sp.off[s];
@@ -1015,20 +1273,18 @@ int fx_from64(emg_ctx *c, const State64 &src) {
   return bucket_ranges(c, d, c->sp.nspec, 2048, flat);
 }
 
-// bucketed FP32 (two runs per bucket, spill) -> one run per bucket, no spill
+// bucketed FP32 (runs, outboxes, spill) -> one run per bucket, nothing pending
 int fx_rebucket(emg_ctx *c) {
-  return bucket_ranges(c, bucket_desc(c), c->fx.nb + c->sp.nspec, 1, c->fx.cur);
+  return bucket_ranges(c, bucket_desc(c), 2 * c->fx.nb + c->sp.nspec, 1, c->fx.cur);
 }
 
 int fx_to64(emg_ctx *c, State64 &dst) {
   FxBuckets &B = c->fx;
-  // spill regions are absorbed first so every particle sits in a bucket
   int rc = fx_rebucket(c);
   if (rc) return rc;
   exclusive_scan_i32(B.count[B.cur], B.start[1 - B.cur], B.nb, c->scan_tmp, c->stream);
   hipLaunchKernelGGL(k_fx_to64, dim3(B.nb), dim3(kBlock), 0, c->stream, B.st[B.cur],
-                     B.start[B.cur], B.count[B.cur], B.narr[B.cur], B.cap[B.cur],
-                     B.start[1 - B.cur], B.ntiles, c->sp, dst);
+                     B.start[B.cur], B.count[B.cur], B.start[1 - B.cur], B.ntiles, c->sp, dst);
   return hip_check(hipGetLastError(), "fx_to64");
 }
 
@@ -1037,9 +1293,9 @@ void launch_fx_step(emg_ctx *c, int ustep, bool push, bool deposit, bool em) {
   FxBuckets &B = c->fx;
   int in = B.cur, out = 1 - B.cur;
   unsigned gb = (unsigned)((B.nb + kBlock - 1) / kBlock);
-  hipLaunchKernelGGL(k_fx_caps, dim3(gb), dim3(kBlock), 0, c->stream, B.count[in], B.cap[out],
-                     B.spill[out], B.nb, c->sp.nspec);
-  hipLaunchKernelGGL(k_fill_i32, dim3(gb), dim3(kBlock), 0, c->stream, B.back, B.nb, 0);
+  hipLaunchKernelGGL(k_fx_caps, dim3(gb), dim3(kBlock), 0, c->stream, B.count[in], B.pre[in],
+                     B.cap[out], B.extra, B.spill[out], B.nb, c->sp.nspec, B.ntx, B.nty, B.ntz,
+                     B.ntiles, near_ok(B));
   exclusive_scan_i32(B.cap[out], B.start[out], B.nb, c->scan_tmp, c->stream);
   if (deposit) launch_zero(c->rhofx, sizeof(unsigned long long) * c->ncell * c->sp.nspec, c->stream);
   FxArgs a = make_args(c);
@@ -1051,9 +1307,9 @@ void launch_fx_step(emg_ctx *c, int ustep, bool push, bool deposit, bool em) {
     case 8: step_t<8>(c, a, push, deposit, grid); break;
   }
   timer_end(c, kTPush);
-  hipLaunchKernelGGL(k_fx_finish, dim3(gb), dim3(kBlock), 0, c->stream, B.front, B.back,
-                     B.count[in], B.cap[out], B.narr[out], B.count[out], B.nb, B.spill[out],
-                     c->sp.nspec, (int32_t)B.spill_cap);
+  hipLaunchKernelGGL(k_fx_finish, dim3(gb), dim3(kBlock), 0, c->stream, B.stay, B.extra,
+                     B.nst[out], B.count[out], B.nb, B.spill[out], c->sp.nspec,
+                     (int32_t)B.spill_cap);
   B.cur = out;
 }
 
@@ -1072,10 +1328,9 @@ void launch_fx_deposit(emg_ctx *c) {
 
 void launch_fx_energy(emg_ctx *c) {
   FxBuckets &B = c->fx;
+  FxArgs a = make_args(c);
   timer_begin(c, kTEnergy);
-  hipLaunchKernelGGL(k_fx_energy, dim3(B.nb + c->sp.nspec), dim3(kBlock), 0, c->stream,
-                     B.st[B.cur], B.start[B.cur], B.count[B.cur], B.narr[B.cur], B.cap[B.cur], B.nb,
-                     B.spill[B.cur], B.area, B.spill_cap, c->ke_part);
+  hipLaunchKernelGGL(k_fx_energy, dim3(B.nb + c->sp.nspec), dim3(kBlock), 0, c->stream, a);
   hipLaunchKernelGGL(k_ke_final, dim3(c->sp.nspec), dim3(kBlock), 0, c->stream, c->ke_part,
                      B.ntiles, B.nb, 1, c->ke_out);
   timer_end(c, kTEnergy);

@@ -216,6 +216,60 @@ def test_fp32_fixed_one_step_vs_fp64_oracle(tile):
         assert np.abs(rho32[k] - ok).max() <= 1e-6 * np.abs(ok).max(), k
 
 
+def test_fp32_fixed_multistep_with_tile_crossings():
+    """Six FP32 steps on a grid with at least three tiles per axis, fast enough
+    that a few percent of the particles change tile every step: movers travel
+    through the outboxes and are pulled by their new tiles.  Every particle is
+    kept, the trajectories follow the FP64 oracle, and the standalone deposit
+    and energy of the final state (which includes the arrivals still waiting
+    in outboxes) match the oracle's."""
+    n = (32, 24, 16)
+    counts = (60000, 30000)
+    q, qm, rm = (0.002, -0.002), (1.0, -0.01), (0.002, 0.2)
+    rng = np.random.default_rng(11)
+    eb = periodic_eb(n, rng, 0.02, 0.005)
+    b0 = (0.0, 0.01, 0.02)
+    vth = (0.3, 0.02)
+    p = mt_particles(n, counts, vth)
+    nsteps = 6
+    with ctx_for(n, q, qm, rm, counts, precision=emg.FP32_FIXED, b0=b0, tile=4,
+                 track_pid=True) as ctx:
+        ctx.upload(p, counts)
+        start, _ = ctx.download()
+        ctx.set_fields(eb)
+        for _ in range(nsteps):
+            ctx.push_es(2)
+        ctx.deposit_rho(0)
+        rho32 = [ctx.rho(species=k) for k in range(2)]
+        ek32 = ctx.energy()
+        g, cnt = ctx.download()
+    assert list(cnt) == list(counts)
+    o = by_pid(start)
+    g = by_pid(g)
+    assert np.array_equal(g["pid"], o["pid"])
+    moved = 0
+    wrk = oracle.grdcrt(eb, n, (0, 0, 0), b0)
+    for _ in range(nsteps):
+        before = np.floor(o["x"] / 4) + 100 * np.floor(o["y"] / 4) + 1e4 * np.floor(o["z"] / 4)
+        oracle.psolve2(o, counts, qm, wrk, n, 2, np.array(n, dtype=float))
+        after = np.floor(o["x"] / 4) + 100 * np.floor(o["y"] / 4) + 1e4 * np.floor(o["z"] / 4)
+        moved += int(np.count_nonzero(before != after))
+    assert moved > 0.02 * nsteps * sum(counts)   # the outbox path carries real traffic
+    for f, L in zip(("x", "y", "z"), n):
+        d = np.abs(g[f] - o[f])
+        assert np.minimum(d, L - d).max() <= 1e-4, f
+    vt = np.array(vth)[np.repeat([0, 1], counts)]
+    for f in ("vx", "vy", "vz"):
+        assert (np.abs(g[f] - o[f]) / np.maximum(np.abs(o[f]), vt)).max() <= 1e-4, f
+    _, rhodg = oracle.charge(o, counts, q, n, with_species=True)
+    oracle.fold_rho(rhodg, n, ncomp=2)
+    for k in range(2):
+        ok = oracle.rho_nodes(rhodg)[..., k]
+        assert np.abs(rho32[k] - ok).max() <= 1e-5 * np.abs(ok).max(), k
+    ok = oracle.energy(o, counts, rm, np.array(n, dtype=float))
+    assert np.allclose(ek32, ok, rtol=1e-5)
+
+
 def test_fixed_point_rho_is_order_and_tile_independent():
     n = (16, 16, 16)
     counts = (50000, 20000)
@@ -369,9 +423,9 @@ def test_device_init_statistics():
 
 
 def test_fp32_spill_path_keeps_every_particle():
-    """Crowd 600 particles into a nearly empty tile in one step: the output
-    bucket overflows (capacity count + count/4 + 64), the extras go to the
-    spill region, the next step processes them from there; nothing is lost and
+    """Move 600 particles of one tile into its neighbour in one step: 512 fit
+    the outbox, the rest are placed directly (the destination's 16 end slots,
+    then the spill region, which the next step processes); nothing is lost and
     the result matches the FP64 oracle."""
     n = (16, 16, 16)
     k = 600
