@@ -19,6 +19,9 @@ reference); inputs are resident in HBM before the timed region.
 Prints one JSON line.  `roofline` is the fused FP32 kernel: algorithmic bytes
 per launch (56 B per particle + 24 B per field node + 8 B per charge node and
 species) over its average launch time from HIP events on the context stream.
+`roofline.traffic` is the HBM traffic per launch measured with rocprofv3
+FETCH_SIZE / WRITE_SIZE counters on the same workload (profiles/, read here
+when it was measured for this workload and tile; else null).
 `cpu_baseline` times the oracle (FP64 C restatement of psolve2 + charge +
 fold, one thread) on a bounded sample of the same workload.
 """
@@ -35,6 +38,23 @@ sys.path.insert(0, ROOT)
 
 METRIC = "particle-steps/s push+deposit, FP32/fixed-pt vs FP64 GPU; adjoint/forward time"
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_fx_step_mag128_pmc.txt")
+
+
+def measured_traffic(workload, tile):
+    """HBM bytes per launch of the step kernel from the committed PMC summary
+    (lines `key = value`), if it was measured for this workload and tile."""
+    try:
+        kv = {}
+        for line in open(TRAFFIC_FILE):
+            if "=" in line and not line.lstrip().startswith("#"):
+                k, v = line.split("=", 1)
+                kv[k.strip()] = v.strip()
+        if kv.get("workload") != workload or int(kv.get("tile", -1)) != tile:
+            return None
+        return float(kv["traffic_bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def log(*a):
@@ -121,8 +141,8 @@ def main():
     ap.add_argument("--workload", default="mag128_2sp_64ppc")
     ap.add_argument("--tile", type=int, default=4)
     ap.add_argument("--fp64-steps", type=int, default=None)
-    ap.add_argument("--cpu-sample", type=int, default=12_582_912)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-sample", type=int, default=16_777_216)
+    ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     args = ap.parse_args()
@@ -216,8 +236,9 @@ def main():
                    "kinetic_energy_fp32": ek32.tolist(),
                    "fp64": fp64},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_fx_step<4,true,true> (push + deposit + re-bucket)",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(w.name, args.tile),
+                     "kernel": f"k_fx_step<{args.tile},true,1> (push + deposit + re-bucket)",
                      "kernel_ms": kern_ms, "bytes_per_launch": bytes_launch},
         "cpu_baseline": cpu,
     }

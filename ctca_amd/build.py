@@ -26,6 +26,7 @@ SOURCES = [
     ("k_fp64.hip", ["-ffp-contract=off"]),
     ("k_common.hip", []),
     ("k_fx.hip", []),
+    ("k_adj.hip", []),
     ("emg_api.hip", []),
 ]
 HOST_SOURCES = [("host_init.cpp", ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-I", INC])]

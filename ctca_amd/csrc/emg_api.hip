@@ -314,6 +314,12 @@ int emg_destroy(emg_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   free_state64(c->s64);
   free_state64(c->t64);
+  free_state64(c->lam);
+  for (int k = 0; k < kStateSlots; k++) free_state64(c->slot[k]);
+  dfree(c->adj_e);
+  dfree(c->adj_g);
+  dfree(c->adj_q);
+  dfree(c->adj_sum);
   FxBuckets &B = c->fx;
   for (int k = 0; k < 2; k++) {
     dfree(B.st[k].cell);
@@ -689,12 +695,13 @@ int emg_push_em(emg_ctx *c, int32_t ustep) {
   if (ustep < 1) return set_error(EMG_ERR_ARG, "ustep must be >= 1");
   if (c->cfg.precision == EMG_PLAIN_FP64) {
     launch_push_em64(c, ustep);
+    c->ke_valid = false;
   } else {
-    return set_error(EMG_ERR_UNSUPPORTED, "EM push in EMG_FP32_FIXED is not available yet");
+    launch_fx_step(c, ustep, true, false, true);
+    c->ke_valid = true;  // fused partials of the new velocities
   }
   c->j_valid = true;
   c->rho_valid = false;
-  c->ke_valid = false;
   return hip_check(hipGetLastError(), "push_em launch");
 }
 
@@ -702,6 +709,125 @@ int emg_get_current(emg_ctx *c, double *aj, const int32_t lo[3], const int32_t h
   CTX_CHECK(c);
   if (!c->j_valid) return set_error(EMG_ERR_STATE, "no current deposited");
   return read_grid(c, aj, lo, hi, 3, -1, true);
+}
+
+// ---- adjoint window (FP64 form)
+static int need_fp64(emg_ctx *c) {
+  if (c->cfg.precision != EMG_PLAIN_FP64)
+    return set_error(EMG_ERR_UNSUPPORTED, "the adjoint window runs in EMG_PLAIN_FP64 contexts");
+  return EMG_OK;
+}
+
+static int copy_state(const State64 &src, const State64 &dst, int64_t n, hipStream_t st) {
+  const void *from[7] = {src.x, src.y, src.z, src.vx, src.vy, src.vz, src.pid};
+  void *to[7] = {dst.x, dst.y, dst.z, dst.vx, dst.vy, dst.vz, dst.pid};
+  for (int q = 0; q < 7; q++) {
+    size_t bytes = (q < 6 ? sizeof(double) : sizeof(int32_t)) * (size_t)n;
+    int rc = hip_check(hipMemcpyAsync(to[q], from[q], bytes, hipMemcpyDeviceToDevice, st),
+                       "state copy");
+    if (rc) return rc;
+  }
+  return EMG_OK;
+}
+
+static int need_adjoint(emg_ctx *c) {
+  int rc = need_fp64(c);
+  if (rc) return rc;
+  if (c->lam.x) return EMG_OK;
+  if ((rc = alloc_state64(c->lam, c->cap_total)) ||
+      (rc = dalloc(&c->adj_e, 3 * c->ncell, "adjoint field")) ||
+      (rc = dalloc(&c->adj_g, c->ncell, "adjoint density")) ||
+      (rc = dalloc(&c->adj_q, EMG_MAX_SPECIES, "adjoint factors")) ||
+      (rc = dalloc(&c->adj_sum, 3 * EMG_MAX_SPECIES, "adjoint sums")))
+    return rc;
+  State64 &L = c->lam;
+  double *arr[6] = {L.x, L.y, L.z, L.vx, L.vy, L.vz};
+  for (double *p : arr) launch_zero(p, sizeof(double) * c->cap_total, c->stream);
+  launch_zero(c->adj_e, sizeof(double) * 3 * c->ncell, c->stream);
+  return hip_check(hipGetLastError(), "adjoint allocation");
+}
+
+int emg_state_save(emg_ctx *c, int32_t slot) {
+  CTX_CHECK(c);
+  int rc = need_fp64(c);
+  if (!rc) rc = need_particles(c);
+  if (rc) return rc;
+  if (slot < 0 || slot >= kStateSlots) return set_error(EMG_ERR_ARG, "slot must be 0..63");
+  if (!c->slot[slot].x && (rc = alloc_state64(c->slot[slot], c->cap_total))) return rc;
+  for (int s = 0; s < c->sp.nspec; s++) c->slot_cnt[slot][s] = c->sp.cnt[s];
+  return copy_state(c->s64, c->slot[slot], c->cap_total, c->stream);
+}
+
+int emg_state_load(emg_ctx *c, int32_t slot) {
+  CTX_CHECK(c);
+  int rc = need_fp64(c);
+  if (rc) return rc;
+  if (slot < 0 || slot >= kStateSlots || !c->slot[slot].x)
+    return set_error(EMG_ERR_ARG, "slot holds no saved state");
+  for (int s = 0; s < c->sp.nspec; s++) c->sp.cnt[s] = c->slot_cnt[slot][s];
+  c->have_particles = true;
+  c->rho_valid = c->j_valid = c->ke_valid = false;
+  return copy_state(c->slot[slot], c->s64, c->cap_total, c->stream);
+}
+
+int emg_adj_reset(emg_ctx *c) {
+  CTX_CHECK(c);
+  int rc = need_adjoint(c);
+  if (rc) return rc;
+  State64 &L = c->lam;
+  double *arr[6] = {L.x, L.y, L.z, L.vx, L.vy, L.vz};
+  for (double *p : arr) launch_zero(p, sizeof(double) * c->cap_total, c->stream);
+  return hip_check(hipGetLastError(), "adjoint reset");
+}
+
+int emg_adj_step(emg_ctx *c, int32_t ustep) {
+  CTX_CHECK(c);
+  int rc = need_adjoint(c);
+  if (!rc) rc = need_particles(c);
+  if (!rc) rc = need_fields(c);
+  if (rc) return rc;
+  if (ustep < 0) return set_error(EMG_ERR_ARG, "ustep must be >= 0");
+  launch_adj_step64(c, ustep);
+  return hip_check(hipGetLastError(), "adjoint step launch");
+}
+
+int emg_adj_get_field(emg_ctx *c, double *lam_e) {
+  CTX_CHECK(c);
+  if (!lam_e) return set_error(EMG_ERR_ARG, "null argument");
+  int rc = need_adjoint(c);
+  if (!rc) rc = check_flags(c);
+  if (rc) return rc;
+  return hip_check(hipMemcpy(lam_e, c->adj_e, sizeof(double) * 3 * c->ncell, hipMemcpyDeviceToHost),
+                   "adjoint field copy");
+}
+
+int emg_adj_add_density(emg_ctx *c, const double *g, const double *qfac) {
+  CTX_CHECK(c);
+  if (!g || !qfac) return set_error(EMG_ERR_ARG, "null argument");
+  int rc = need_adjoint(c);
+  if (!rc) rc = need_particles(c);
+  if (rc) return rc;
+  if ((rc = hip_check(hipMemcpyAsync(c->adj_g, g, sizeof(double) * c->ncell, hipMemcpyHostToDevice,
+                                     c->stream), "adjoint density upload")) ||
+      (rc = hip_check(hipMemcpyAsync(c->adj_q, qfac, sizeof(double) * c->sp.nspec,
+                                     hipMemcpyHostToDevice, c->stream), "adjoint factor upload")))
+    return rc;
+  launch_adj_density64(c, c->adj_q);
+  // the host buffers may be reused once this returns
+  return hip_check(hipStreamSynchronize(c->stream), "adjoint density");
+}
+
+int emg_adj_velocity_sums(emg_ctx *c, double *out) {
+  CTX_CHECK(c);
+  if (!out) return set_error(EMG_ERR_ARG, "null argument");
+  int rc = need_adjoint(c);
+  if (!rc) rc = need_particles(c);
+  if (rc) return rc;
+  launch_adj_vsum(c, c->adj_sum);
+  rc = check_flags(c);
+  if (rc) return rc;
+  return hip_check(hipMemcpy(out, c->adj_sum, sizeof(double) * 3 * c->sp.nspec,
+                             hipMemcpyDeviceToHost), "adjoint sums copy");
 }
 
 int emg_sort(emg_ctx *c) {

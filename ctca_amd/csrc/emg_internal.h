@@ -14,10 +14,11 @@ namespace emg {
 constexpr int kBlock = 256;
 constexpr int kFixShift = 24;  // rho fixed point: unit CIC weight = 2^24
 constexpr double kFixScale = 16777216.0;
-constexpr int kJShift = 32;  // current fixed point: 2^32 per (cell / unit time)
-constexpr double kJScale = 4294967296.0;
+constexpr int kJShift = 24;  // current fixed point: 2^24 per unit flux (cell / unit time)
+constexpr double kJScale = 16777216.0;
 constexpr int kCellBits = 10;  // packed cell index: ix | iy << 10 | iz << 20
 constexpr int kCellMask = (1 << kCellBits) - 1;
+constexpr int kStateSlots = 64;  // saved particle states of the adjoint window
 
 // device error flags
 constexpr int kFlagDomain = 1;    // particle outside [0, L] / non-finite
@@ -143,6 +144,15 @@ struct emg_ctx {
   int *dflag;       // device error flags
   int *hflag;       // pinned host mirror
 
+  // adjoint window (FP64 form)
+  emg::State64 lam;  // lambda_x in x, y, z; lambda_v in vx, vy, vz
+  double *adj_e;     // 3 x ncell: transposed gather of qmp * lambda_a
+  double *adj_g;     // ncell: uploaded density adjoint
+  double *adj_q;     // nspec: its factor per species
+  double *adj_sum;   // 3 x nspec: velocity adjoint sums
+  emg::State64 slot[emg::kStateSlots];  // saved particle states
+  int64_t slot_cnt[emg::kStateSlots][EMG_MAX_SPECIES];
+
   bool timing;
   emg::Timer timers[emg::kTNum];
 };
@@ -176,6 +186,11 @@ void launch_read_current(emg_ctx *c, double *dst_dev, const int lo[3], const int
 void launch_zero(void *p, size_t bytes, hipStream_t s);
 int sort64(emg_ctx *c);
 int init_device(emg_ctx *c, const emg_device_species *spec, int nspec, uint64_t seed);
+
+// adjoint of the FP64 ES push (k_adj.hip)
+void launch_adj_step64(emg_ctx *c, int ustep);
+void launch_adj_density64(emg_ctx *c, const double *qfac_dev);
+void launch_adj_vsum(emg_ctx *c, double *out_dev);
 
 // FP32 + fixed point (k_fx.hip)
 int fx_alloc(emg_ctx *c, int64_t ntotal);

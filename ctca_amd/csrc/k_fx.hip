@@ -73,10 +73,11 @@ struct FxArgs {
   int64_t area, spill_cap;
   int32_t nrec;  // bytes per particle array of a buffer (bound of the masked stores)
   const float *node32;
-  unsigned long long *rho;
+  unsigned long long *rho;   // charge, nspec x ncell
+  unsigned long long *jcur;  // zigzag current, nspec x 3 x ncell
   int64_t ncell;
   float qm[EMG_MAX_SPECIES];
-  float gstep;
+  float gstep, rustep;
   double *ke_part;  // [(nb + nspec * kSpillBlocksPerSpecies) * 3]
   int *flag;
 };
@@ -340,6 +341,91 @@ __device__ void place_direct(const FxArgs &a, int dest, int cell, int pid, float
   store_pcl(a.out, slot, cell, pid, fx, fy, fz, vx, vy, vz);
 }
 
+// Umeda zigzag current of one particle (psolve1.F90:399-415 relay point,
+// :717-789 segments and weights), in the frame of the old cell: the particle
+// goes from fb (offset in the old cell) to d + fa (d whole cells, fa the new
+// offset); the relay point is the midpoint when the cell does not change, else
+// the face between the two cells.  Per axis: the offset of each segment's
+// midpoint in its own cell and the segment's flux (displacement / ustep).
+struct Zig {
+  float md1, vv1, md2, vv2;
+};
+
+__device__ __forceinline__ Zig zig_axis(float fb, int d, float fa, float rustep) {
+  float xa = (float)d + fa;
+  float r = d == 0 ? 0.5f * (fb + xa) : (float)max(0, d);
+  Zig z;
+  z.md1 = 0.5f * (r + fb);
+  z.vv1 = (r - fb) * rustep;
+  z.md2 = 0.5f * (xa + r) - (float)d;
+  z.vv2 = (xa - r) * rustep;
+  return z;
+}
+
+// the four weights of a flux component over the transverse midpoint offsets
+// (u, w): the reference's arithmetic form (psolve1.F90:728-734)
+__device__ __forceinline__ void zig_w4(float u, float w, float W[4]) {
+  W[3] = u * w;
+  W[1] = u - W[3];
+  W[2] = w - W[3];
+  W[0] = 1.f - u - W[2];
+}
+
+// fixed point of a current contribution (kJScale per unit flux; a flux is at
+// most a few cells per unit time, far inside the int32 range)
+__device__ __forceinline__ unsigned long long jfix(float v) {
+  return (unsigned long long)(long long)(int)rintf(v * (float)kJScale);
+}
+
+// Add the 12 contributions of one segment at cell base q (LDS tile, N3 nodes
+// per axis) or at cell (i, j, k) of the global accumulators.
+template <int N3>
+__device__ __forceinline__ void zig_seg_lds(unsigned long long *jh, int q, float mx, float my,
+                                            float mz, float vx, float vy, float vz) {
+  constexpr int P = N3 * N3 * N3;
+  float W[4];
+  zig_w4(my, mz, W);  // Jx over (y, z)
+  atomicAdd(&jh[q], jfix(vx * W[0]));
+  atomicAdd(&jh[q + N3], jfix(vx * W[1]));
+  atomicAdd(&jh[q + N3 * N3], jfix(vx * W[2]));
+  atomicAdd(&jh[q + N3 + N3 * N3], jfix(vx * W[3]));
+  zig_w4(mx, mz, W);  // Jy over (x, z)
+  atomicAdd(&jh[P + q], jfix(vy * W[0]));
+  atomicAdd(&jh[P + q + 1], jfix(vy * W[1]));
+  atomicAdd(&jh[P + q + N3 * N3], jfix(vy * W[2]));
+  atomicAdd(&jh[P + q + 1 + N3 * N3], jfix(vy * W[3]));
+  zig_w4(mx, my, W);  // Jz over (x, y)
+  atomicAdd(&jh[2 * P + q], jfix(vz * W[0]));
+  atomicAdd(&jh[2 * P + q + 1], jfix(vz * W[1]));
+  atomicAdd(&jh[2 * P + q + N3], jfix(vz * W[2]));
+  atomicAdd(&jh[2 * P + q + 1 + N3], jfix(vz * W[3]));
+}
+
+__device__ __forceinline__ void zig_seg_global(unsigned long long *j3, int64_t ncell,
+                                               const Grid &g, int i, int j, int k, float mx,
+                                               float my, float mz, float vx, float vy, float vz) {
+  int i1 = i + 1 == g.nx ? 0 : i + 1, j1 = j + 1 == g.ny ? 0 : j + 1, k1 = k + 1 == g.nz ? 0 : k + 1;
+  auto at3 = [&](int d, int a, int b, int c) {
+    return &j3[(int64_t)d * ncell + ((int64_t)c * g.ny + b) * g.nx + a];
+  };
+  float W[4];
+  zig_w4(my, mz, W);
+  atomicAdd(at3(0, i, j, k), jfix(vx * W[0]));
+  atomicAdd(at3(0, i, j1, k), jfix(vx * W[1]));
+  atomicAdd(at3(0, i, j, k1), jfix(vx * W[2]));
+  atomicAdd(at3(0, i, j1, k1), jfix(vx * W[3]));
+  zig_w4(mx, mz, W);
+  atomicAdd(at3(1, i, j, k), jfix(vy * W[0]));
+  atomicAdd(at3(1, i1, j, k), jfix(vy * W[1]));
+  atomicAdd(at3(1, i, j, k1), jfix(vy * W[2]));
+  atomicAdd(at3(1, i1, j, k1), jfix(vy * W[3]));
+  zig_w4(mx, my, W);
+  atomicAdd(at3(2, i, j, k), jfix(vz * W[0]));
+  atomicAdd(at3(2, i1, j, k), jfix(vz * W[1]));
+  atomicAdd(at3(2, i, j1, k), jfix(vz * W[2]));
+  atomicAdd(at3(2, i1, j1, k), jfix(vz * W[3]));
+}
+
 // One particle.  The common case is straight-line code (LDS gather, LDS charge
 // tile, staying in the own bucket or going to the LDS outbox); the rare cases
 // (outside the staged tile, beyond the charge halo, a move of a box length or
@@ -347,7 +433,8 @@ __device__ void place_direct(const FxArgs &a, int dest, int cell, int pid, float
 // lane of the wave needs them.  The stayers' stores are masked stores, so the
 // common path issues the same memory instructions whatever the data.  `act`
 // is false for lanes past the end of a round; full rounds pass a literal true.
-template <int T, bool PUSH, bool DEPOSIT, bool PID>
+// DEP: 0 no deposit, 1 charge (psolve2 + charge), 2 zigzag current (psolve1).
+template <int T, bool PUSH, int DEP, bool PID>
 __device__ __forceinline__ void fx_one(const FxArgs &a, int s, bool act, const Pcl &p, int ox,
                                        int oy, int oz, bool lds_ok, uint32_t stay_base,
                                        int *n_stay, int *n_out, FxRec *outbox, const float4 *fA,
@@ -380,6 +467,8 @@ __device__ __forceinline__ void fx_one(const FxArgs &a, int s, bool act, const P
     ke[1] = fmaf(ay, ay, ke[1]);
     ke[2] = fmaf(az, az, ke[2]);
   }
+  const float fbx = fx, fby = fy, fbz = fz;
+  const int ibx = ix, iby = iy, ibz = iz;
   int dx = move_offset(fx, vx, a.gstep);
   int dy = move_offset(fy, vy, a.gstep);
   int dz = move_offset(fz, vz, a.gstep);
@@ -407,7 +496,7 @@ __device__ __forceinline__ void fx_one(const FxArgs &a, int s, bool act, const P
   int sbase = 0;
   const int sleader = __ffsll((unsigned long long)m) - 1;
   if (m != 0 && lane == sleader) sbase = atomicAdd(n_stay, __popcll(m));
-  if (DEPOSIT) {
+  if (DEP == 1) {
     int W[8];
     fixed_weights(fx, fy, fz, W);
     bool in_halo = in_tile && rmax <= (unsigned)(T + 1);
@@ -432,6 +521,26 @@ __device__ __forceinline__ void fx_one(const FxArgs &a, int s, bool act, const P
           atomicAdd(&r[((int64_t)zs[c2] * g.ny + ys[b2]) * g.nx + xs[a2]],
                     (unsigned long long)(long long)W[q]);
         }
+      }
+    }
+  }
+  if (DEP == 2) {
+    const Zig zx = zig_axis(fbx, dx, fx, a.rustep), zy = zig_axis(fby, dy, fy, a.rustep),
+              zz = zig_axis(fbz, dz, fz, a.rustep);
+    bool in_halo = in_tile && rmax <= (unsigned)(T + 1);
+    if (act && in_halo) {
+      zig_seg_lds<N3>(rh, ((lz + 1) * N3 + (ly + 1)) * N3 + lx + 1, zx.md1, zy.md1, zz.md1,
+                      zx.vv1, zy.vv1, zz.vv1);
+      zig_seg_lds<N3>(rh, (rz * N3 + ry) * N3 + rx, zx.md2, zy.md2, zz.md2, zx.vv2, zy.vv2,
+                      zz.vv2);
+    }
+    if (__ballot(act && !in_halo)) {
+      if (act && !in_halo) {
+        unsigned long long *j3 = a.jcur + (int64_t)s * 3 * a.ncell;
+        zig_seg_global(j3, a.ncell, g, ibx, iby, ibz, zx.md1, zy.md1, zz.md1, zx.vv1, zy.vv1,
+                       zz.vv1);
+        zig_seg_global(j3, a.ncell, g, ix, iy, iz, zx.md2, zy.md2, zz.md2, zx.vv2, zy.vv2,
+                       zz.vv2);
       }
     }
   }
@@ -542,13 +651,14 @@ __device__ __forceinline__ int64_t arrival_rec(const int64_t *seg_src, const int
   return seg_src[lo] + (j - seg_pre[lo]);
 }
 
-template <int T, bool PUSH, bool DEPOSIT, bool PID>
+template <int T, bool PUSH, int DEP, bool PID>
 __global__ __launch_bounds__(kBlock) void k_fx_step(FxArgs a) {
   constexpr int N1 = T + 1, N3 = T + 3;
   constexpr int kPer = kOutbox / kBlock;
   __shared__ float4 fA[PUSH ? N1 * N1 * N1 : 1];
   __shared__ float2 fB[PUSH ? N1 * N1 * N1 : 1];
-  __shared__ unsigned long long rh[DEPOSIT ? N3 * N3 * N3 : 1];
+  constexpr int kTileNodes = DEP == 1 ? N3 * N3 * N3 : (DEP == 2 ? 3 * N3 * N3 * N3 : 1);
+  __shared__ unsigned long long rh[kTileNodes];  // charge tile, or the three current tiles
   __shared__ FxRec outbox[kOutbox];
   __shared__ int64_t seg_src[kCodes];
   __shared__ int seg_pre[kPre];
@@ -581,7 +691,7 @@ __global__ __launch_bounds__(kBlock) void k_fx_step(FxArgs a) {
       load_off<PID>(a.in, off_of(min(i + (int)threadIdx.x, cnt - 1)), p);
     };
     auto one = [&](bool act, const Pcl &p) {
-      fx_one<T, PUSH, DEPOSIT, PID>(a, s, act, p, ox, oy, oz, true, stay_base, &n_stay, &n_out,
+      fx_one<T, PUSH, DEP, PID>(a, s, act, p, ox, oy, oz, true, stay_base, &n_stay, &n_out,
                                     outbox, fA, fB, rh, ke);
     };
     // Software pipelined three deep: while a lane pushes particle k, the loads
@@ -610,8 +720,8 @@ __global__ __launch_bounds__(kBlock) void k_fx_step(FxArgs a) {
         }
       }
     }
-    if (DEPOSIT)
-      for (int q = threadIdx.x; q < N3 * N3 * N3; q += kBlock) rh[q] = 0ull;
+    if (DEP != 0)
+      for (int q = threadIdx.x; q < kTileNodes; q += kBlock) rh[q] = 0ull;
     __syncthreads();
     if (cnt > 0) {
       const int nfull = (cnt / kBlock) * kBlock;
@@ -640,17 +750,21 @@ __global__ __launch_bounds__(kBlock) void k_fx_step(FxArgs a) {
       if (!PID) P.pid = 0;
       one(j < narr, P);
     }
-    if (DEPOSIT) {
+    if (DEP != 0) {
+      // tile node (rx, ry, rz) of component d -> cell (ox - 1 + rx, ...) of
+      // the species' charge (d = 0) or current component d
       __syncthreads();
-      unsigned long long *r = a.rho + (int64_t)s * a.ncell;
-      for (int q = threadIdx.x; q < N3 * N3 * N3; q += kBlock) {
+      unsigned long long *r = DEP == 1 ? a.rho + (int64_t)s * a.ncell
+                                       : a.jcur + (int64_t)s * 3 * a.ncell;
+      for (int q = threadIdx.x; q < kTileNodes; q += kBlock) {
         unsigned long long v = rh[q];
         if (v != 0ull) {
-          int rx = q % N3, ry = (q / N3) % N3, rz = q / (N3 * N3);
+          int d = q / (N3 * N3 * N3), qq = q - d * (N3 * N3 * N3);
+          int rx = qq % N3, ry = (qq / N3) % N3, rz = qq / (N3 * N3);
           int gx = wrapc(ox - 1 + rx, a.g.nx);
           int gy = wrapc(oy - 1 + ry, a.g.ny);
           int gz = wrapc(oz - 1 + rz, a.g.nz);
-          atomicAdd(&r[((int64_t)gz * a.g.ny + gy) * a.g.nx + gx], v);
+          atomicAdd(&r[(int64_t)d * a.ncell + ((int64_t)gz * a.g.ny + gy) * a.g.nx + gx], v);
         }
       }
     }
@@ -669,7 +783,7 @@ __global__ __launch_bounds__(kBlock) void k_fx_step(FxArgs a) {
       int i = (r * kSpillBlocksPerSpecies + sub) * kBlock + threadIdx.x;
       Pcl cur{};
       if (i < cnt) load_pcl(a.in, beg + i, cur);
-      fx_one<T, PUSH, DEPOSIT, PID>(a, s, i < cnt, cur, 0, 0, 0, false, 0u, &n_stay, &n_out,
+      fx_one<T, PUSH, DEP, PID>(a, s, i < cnt, cur, 0, 0, 0, false, 0u, &n_stay, &n_out,
                                     outbox, fA, fB, rh, ke);
     }
   }
@@ -1083,32 +1197,34 @@ FxArgs make_args(emg_ctx *c) {
   a.nrec = (int32_t)(B.cap_total * 4);
   a.node32 = c->node32;
   a.rho = c->rhofx;
+  a.jcur = c->jfx;
   a.ncell = c->ncell;
   for (int s = 0; s < EMG_MAX_SPECIES; s++) a.qm[s] = (float)c->cfg.qm[s];
   a.gstep = 0.f;
+  a.rustep = 0.f;
   a.ke_part = c->ke_part;
   a.flag = c->dflag;
   return a;
 }
 
+// launched forms: push + charge (psolve2 + charge), push + zigzag current
+// (psolve1), move only (positn)
 template <int T, bool PID>
-void step_tp(emg_ctx *c, const FxArgs &a, bool push, bool deposit, unsigned grid) {
-  if (push && deposit)
-    hipLaunchKernelGGL((k_fx_step<T, true, true, PID>), dim3(grid), dim3(kBlock), 0, c->stream, a);
-  else if (push)
-    hipLaunchKernelGGL((k_fx_step<T, true, false, PID>), dim3(grid), dim3(kBlock), 0, c->stream, a);
-  else if (deposit)
-    hipLaunchKernelGGL((k_fx_step<T, false, true, PID>), dim3(grid), dim3(kBlock), 0, c->stream, a);
+void step_tp(emg_ctx *c, const FxArgs &a, bool push, int dep, unsigned grid) {
+  if (push && dep == 1)
+    hipLaunchKernelGGL((k_fx_step<T, true, 1, PID>), dim3(grid), dim3(kBlock), 0, c->stream, a);
+  else if (push && dep == 2)
+    hipLaunchKernelGGL((k_fx_step<T, true, 2, PID>), dim3(grid), dim3(kBlock), 0, c->stream, a);
   else
-    hipLaunchKernelGGL((k_fx_step<T, false, false, PID>), dim3(grid), dim3(kBlock), 0, c->stream, a);
+    hipLaunchKernelGGL((k_fx_step<T, false, 0, PID>), dim3(grid), dim3(kBlock), 0, c->stream, a);
 }
 
 template <int T>
-void step_t(emg_ctx *c, const FxArgs &a, bool push, bool deposit, unsigned grid) {
+void step_t(emg_ctx *c, const FxArgs &a, bool push, int dep, unsigned grid) {
   if (a.in.pid)
-    step_tp<T, true>(c, a, push, deposit, grid);
+    step_tp<T, true>(c, a, push, dep, grid);
   else
-    step_tp<T, false>(c, a, push, deposit, grid);
+    step_tp<T, false>(c, a, push, dep, grid);
 }
 
 // bucket the particles described by d (reading buffer 'src') into buffer 1 - src
@@ -1289,22 +1405,25 @@ int fx_to64(emg_ctx *c, State64 &dst) {
 }
 
 void launch_fx_step(emg_ctx *c, int ustep, bool push, bool deposit, bool em) {
-  (void)em;
   FxBuckets &B = c->fx;
+  const int dep = em ? 2 : (deposit ? 1 : 0);
   int in = B.cur, out = 1 - B.cur;
   unsigned gb = (unsigned)((B.nb + kBlock - 1) / kBlock);
   hipLaunchKernelGGL(k_fx_caps, dim3(gb), dim3(kBlock), 0, c->stream, B.count[in], B.pre[in],
                      B.cap[out], B.extra, B.spill[out], B.nb, c->sp.nspec, B.ntx, B.nty, B.ntz,
                      B.ntiles, near_ok(B));
   exclusive_scan_i32(B.cap[out], B.start[out], B.nb, c->scan_tmp, c->stream);
-  if (deposit) launch_zero(c->rhofx, sizeof(unsigned long long) * c->ncell * c->sp.nspec, c->stream);
+  if (dep == 1) launch_zero(c->rhofx, sizeof(unsigned long long) * c->ncell * c->sp.nspec, c->stream);
+  if (dep == 2)
+    launch_zero(c->jfx, sizeof(unsigned long long) * 3 * c->ncell * c->sp.nspec, c->stream);
   FxArgs a = make_args(c);
   a.gstep = (float)ustep;
+  a.rustep = ustep > 0 ? 1.f / (float)ustep : 0.f;
   unsigned grid = (unsigned)(B.nb + c->sp.nspec * kSpillBlocksPerSpecies);
   timer_begin(c, kTPush);
   switch (B.tile) {
-    case 4: step_t<4>(c, a, push, deposit, grid); break;
-    case 8: step_t<8>(c, a, push, deposit, grid); break;
+    case 4: step_t<4>(c, a, push, dep, grid); break;
+    case 8: step_t<8>(c, a, push, dep, grid); break;
   }
   timer_end(c, kTPush);
   hipLaunchKernelGGL(k_fx_finish, dim3(gb), dim3(kBlock), 0, c->stream, B.stay, B.extra,

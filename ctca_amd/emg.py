@@ -26,6 +26,8 @@ PARTICLE = np.dtype([("x", "f8"), ("y", "f8"), ("z", "f8"), ("vx", "f8"), ("vy",
                      ("vz", "f8"), ("pid", "i4"), ("preside", "i4"), ("nid", "i4"), ("spec", "i4")])
 assert PARTICLE.itemsize == 64
 
+EMG_STATE_SLOTS = 64  # saved particle states per context (emg_state_save)
+
 # every entry point declared in include/emgpu.h
 EXPORTS = (
     "emg_create", "emg_destroy", "emg_last_error", "emg_set_stream", "emg_synchronize",
@@ -33,6 +35,8 @@ EXPORTS = (
     "emg_positn", "emg_push_es", "emg_deposit_rho", "emg_get_rho", "emg_get_rhodg", "emg_energy",
     "emg_push_em", "emg_get_current", "emg_sort", "emg_init_device", "emg_host_inipcl",
     "emg_enable_timing", "emg_kernel_stats", "emg_reset_kernel_stats",
+    "emg_state_save", "emg_state_load", "emg_adj_reset", "emg_adj_step", "emg_adj_get_field",
+    "emg_adj_add_density", "emg_adj_velocity_sums",
 )
 
 
@@ -100,6 +104,13 @@ def lib():
             "emg_enable_timing": [vp, C.c_int32],
             "emg_kernel_stats": [vp, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int64)],
             "emg_reset_kernel_stats": [vp],
+            "emg_state_save": [vp, C.c_int32],
+            "emg_state_load": [vp, C.c_int32],
+            "emg_adj_reset": [vp],
+            "emg_adj_step": [vp, C.c_int32],
+            "emg_adj_get_field": [vp, vp],
+            "emg_adj_add_density": [vp, vp, vp],
+            "emg_adj_velocity_sums": [vp, vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -275,6 +286,37 @@ class Context:
     def energy(self):
         out = np.zeros(3 * self.nspec)
         _chk(lib().emg_energy(self.h, _ptr(out)))
+        return out.reshape(self.nspec, 3)
+
+    # ---- adjoint window (PLAIN_FP64)
+    def state_save(self, slot):
+        _chk(lib().emg_state_save(self.h, int(slot)))
+
+    def state_load(self, slot):
+        _chk(lib().emg_state_load(self.h, int(slot)))
+
+    def adj_reset(self):
+        _chk(lib().emg_adj_reset(self.h))
+
+    def adj_step(self, ustep):
+        _chk(lib().emg_adj_step(self.h, int(ustep)))
+
+    def adj_field(self):
+        """Adjoint of the nodal E: numpy [k, j, i, 3], periodic nodes 0..n-1."""
+        out = np.zeros((self.n[2], self.n[1], self.n[0], 3))
+        _chk(lib().emg_adj_get_field(self.h, _ptr(out)))
+        return out
+
+    def adj_add_density(self, g, qfac):
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        assert g.shape == (self.n[2], self.n[1], self.n[0])
+        q = np.ascontiguousarray(qfac, dtype=np.float64)
+        assert q.shape == (self.nspec,)
+        _chk(lib().emg_adj_add_density(self.h, _ptr(g), _ptr(q)))
+
+    def adj_velocity_sums(self):
+        out = np.zeros(3 * self.nspec)
+        _chk(lib().emg_adj_velocity_sums(self.h, _ptr(out)))
         return out.reshape(self.nspec, 3)
 
     # ---- timing

@@ -215,6 +215,47 @@ module emg_gpu
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
     end function emg_reset_kernel_stats
+    ! adjoint window of the ES push (EMG_PLAIN_FP64 contexts); no reference counterpart
+    integer(c_int) function emg_state_save(ctx, slot) bind(C, name="emg_state_save")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: slot
+    end function emg_state_save
+
+    integer(c_int) function emg_state_load(ctx, slot) bind(C, name="emg_state_load")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: slot
+    end function emg_state_load
+
+    integer(c_int) function emg_adj_reset(ctx) bind(C, name="emg_adj_reset")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function emg_adj_reset
+
+    integer(c_int) function emg_adj_step(ctx, ustep) bind(C, name="emg_adj_step")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: ustep
+    end function emg_adj_step
+
+    integer(c_int) function emg_adj_get_field(ctx, lam_e) bind(C, name="emg_adj_get_field")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), dimension(*), intent(out) :: lam_e
+    end function emg_adj_get_field
+
+    integer(c_int) function emg_adj_add_density(ctx, g, qfac) bind(C, name="emg_adj_add_density")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), dimension(*), intent(in) :: g, qfac
+    end function emg_adj_add_density
+
+    integer(c_int) function emg_adj_velocity_sums(ctx, out) bind(C, name="emg_adj_velocity_sums")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), dimension(*), intent(out) :: out
+    end function emg_adj_velocity_sums
   end interface
 
 contains

@@ -147,7 +147,9 @@ int emg_energy(emg_ctx *ctx, double *engk);
  * species, aj += q(is) * J(is), followed by oh3_exchange_borders(aj, CAJ) +
  * add_boundary_current (psolve1.F90:859-1079) as a periodic fold.  The
  * current accumulators are cleared by each call (psuper1 zeroes aj,
- * src/pcl/psuper1.F90:88-91).
+ * src/pcl/psuper1.F90:88-91).  In EMG_FP32_FIXED the push, move and zigzag
+ * current run as one pass over the tile buckets, the current accumulated per
+ * species in int64 fixed point (2^24 per unit flux).
  */
 int emg_push_em(emg_ctx *ctx, int32_t ustep);
 int emg_get_current(emg_ctx *ctx, double *aj, const int32_t lo[3], const int32_t hi[3]);
@@ -196,6 +198,34 @@ typedef struct {
 
 int emg_host_inipcl(const emg_host_species *spec, int32_t nspec, const int32_t n[3],
                     const double t[9], uint32_t seed, emg_particle *out);
+
+/*
+ * Adjoint window of the electrostatic push (EMG_PLAIN_FP64 contexts).  No
+ * counterpart in the reference: these serve callers that differentiate a
+ * window of ES steps (the torch harness ctca_amd/adjoint.py) with the
+ * reverse step written by hand.
+ *
+ * emg_state_save / emg_state_load copy the particle state to / from one of 64
+ * device slots (checkpoints and the recomputed states of a window).
+ * emg_adj_reset sets the particle adjoints lambda_x = lambda_v = 0.
+ * emg_adj_step is the reverse of one emg_push_es(ustep) taken at the current
+ * particle state and fields: it maps the adjoints of the state after the step
+ * to those of the state before it, except for the state's own effect on the
+ * fields, and leaves the adjoint of the nodal E (the transpose of the gather,
+ * the particles' qmp * lambda_a) for emg_adj_get_field: lam_e(3, 0:nx-1,
+ * 0:ny-1, 0:nz-1), periodic nodes.  emg_adj_add_density adds
+ * qfac[is] * grad g(x) to lambda_x, g(0:nx-1, 0:ny-1, 0:nz-1) a periodic nodal
+ * scalar: the adjoint of the CIC charge (qfac = q) or of any linear function
+ * of the density.  emg_adj_velocity_sums gives sum lambda_v per species and
+ * component (out[3 * is + d]).
+ */
+int emg_state_save(emg_ctx *ctx, int32_t slot);
+int emg_state_load(emg_ctx *ctx, int32_t slot);
+int emg_adj_reset(emg_ctx *ctx);
+int emg_adj_step(emg_ctx *ctx, int32_t ustep);
+int emg_adj_get_field(emg_ctx *ctx, double *lam_e);
+int emg_adj_add_density(emg_ctx *ctx, const double *g, const double *qfac);
+int emg_adj_velocity_sums(emg_ctx *ctx, double *out);
 
 /*
  * Kernel timing with HIP events recorded on the context stream around every

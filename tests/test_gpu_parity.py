@@ -349,6 +349,75 @@ def test_push_em_fp64_vs_oracle():
     assert np.abs(J - oj).max() <= 1e-12 * np.abs(oj).max()
 
 
+@pytest.mark.parametrize("tile,n", [(4, (16, 12, 12)), (4, (12, 8, 8)), (8, (16, 16, 8))])
+def test_push_em_fp32_fixed_vs_fp64_oracle(tile, n):
+    """FP32 zigzag push: one step against the oracle's psolve1 from the same
+    (FP32-rounded) state.  Positions 1e-5 cells, velocities 1e-5 relative, the
+    folded fixed-point current 1e-5 of its maximum.  The grids cover the
+    outbox path (three or more tiles per axis) and the direct path (two)."""
+    counts = (30000, 8000)
+    q, qm, rm = (0.002, 0.002), (1.0, 1.0), (0.002, 0.002)
+    rng = np.random.default_rng(5)
+    eb = periodic_eb(n, rng, 0.02, 0.01)
+    b0 = (0.0, 0.0, 0.02)
+    p = mt_particles(n, counts, (0.15, 0.02), drift=(0.0, 0.15))
+    with ctx_for(n, q, qm, rm, counts, precision=emg.FP32_FIXED, b0=b0, tile=tile,
+                 track_pid=True) as ctx:
+        ctx.upload(p, counts)
+        start, _ = ctx.download()
+        ctx.set_fields(eb)
+        ctx.push_em(2)
+        J = ctx.current(lo=(-1, -1, -1), hi=tuple(v + 1 for v in n))
+        g, cnt = ctx.download()
+    assert list(cnt) == list(counts)
+    o = by_pid(start)
+    g = by_pid(g)
+    wrk = oracle.grdcrt(eb, n, (0, 0, 0), b0)
+    aj = oracle.psolve1(o, counts, qm, q, wrk, n, 2, np.array(n, dtype=float))
+    oracle.fold_current(aj, n)
+    for f, L in zip(("x", "y", "z"), n):
+        d = np.abs(g[f] - o[f])
+        assert np.minimum(d, L - d).max() <= 1e-5, f
+    vt = np.array([0.15, 0.02])[np.repeat([0, 1], counts)]
+    for f in ("vx", "vy", "vz"):
+        assert (np.abs(g[f] - o[f]) / np.maximum(np.abs(o[f]), vt)).max() <= 1e-5, f
+    oj = aj[3:n[2] + 6, 3:n[1] + 6, 3:n[0] + 6]
+    assert np.abs(J - oj).max() <= 1e-5 * np.abs(oj).max()
+
+
+def test_push_em_fp32_fixed_multistep_current():
+    """Five FP32 zigzag steps with tile crossings (outboxes carry the movers);
+    the current of the last step matches the oracle's psolve1 run on the same
+    trajectory to 1e-4 of its maximum, and the kinetic energy to 1e-5."""
+    n = (24, 16, 16)
+    counts = (50000, 10000)
+    q, qm, rm = (0.002, 0.002), (1.0, 1.0), (0.002, 0.002)
+    rng = np.random.default_rng(9)
+    eb = periodic_eb(n, rng, 0.01, 0.005)
+    b0 = (0.01, 0.0, 0.02)
+    p = mt_particles(n, counts, (0.3, 0.05), drift=(0.1, 0.0))
+    with ctx_for(n, q, qm, rm, counts, precision=emg.FP32_FIXED, b0=b0, tile=4,
+                 track_pid=True) as ctx:
+        ctx.upload(p, counts)
+        start, _ = ctx.download()
+        ctx.set_fields(eb)
+        for _ in range(5):
+            ctx.push_em(2)
+        J = ctx.current(lo=(0, 0, 0), hi=tuple(v - 1 for v in n))
+        ek = ctx.energy()
+        _, cnt = ctx.download()
+    assert list(cnt) == list(counts)
+    o = by_pid(start)
+    wrk = oracle.grdcrt(eb, n, (0, 0, 0), b0)
+    sl = np.array(n, dtype=float)
+    for _ in range(5):
+        aj = oracle.psolve1(o, counts, qm, q, wrk, n, 2, sl)
+    oracle.fold_current(aj, n)
+    oj = oracle.aj_cells(aj)
+    assert np.abs(J - oj).max() <= 1e-4 * np.abs(oj).max()
+    assert np.allclose(ek, oracle.energy(o, counts, rm, sl), rtol=1e-5)
+
+
 def test_em_gpu_current_satisfies_planar_continuity():
     n = (12, 10, 8)
     counts = (30000,)
