@@ -133,6 +133,39 @@ def cpu_baseline(w, su, eb, nsample, steps):
                       f"charge + periodic fold, FP64, 1 thread, {t1 - t0:.1f} s"}
 
 
+def adjoint_window(steps, checkpoint):
+    """The adjoint-window configuration (64^3 ES, electrons 32 ppc + a 5% beam
+    at v_d = 3.0, E0 = 0.01, K steps, checkpoints every `checkpoint`): time
+    of the forward window and of the reverse sweep (which recomputes each
+    window's states), and the gradients against central differences."""
+    from ctca_amd import adjoint, emg, units
+    w = units.adjoint64()
+    win = adjoint.ESWindow(w, steps=steps, checkpoint=checkpoint)
+    try:
+        vd, e0 = w.species[1].drift[0], w.e0[0]
+        win.forward(vd, e0)                      # warm-up (allocations, first launches)
+        win.ctx.synchronize()
+        t0 = time.perf_counter()
+        loss = win.forward(vd, e0)
+        win.ctx.synchronize()
+        t1 = time.perf_counter()
+        g_vd, g_e0 = win.backward()
+        win.ctx.synchronize()
+        t2 = time.perf_counter()
+        hv, he = 3e-6, 1e-6
+        fd_vd = (win.forward(vd + hv, e0) - win.forward(vd - hv, e0)) / (2 * hv)
+        fd_e0 = (win.forward(vd, e0 + he) - win.forward(vd, e0 - he)) / (2 * he)
+    finally:
+        win.close()
+    return {"workload": w.name, "particles": int(win.counts.sum()), "steps": steps,
+            "checkpoint_every": checkpoint, "loss": loss,
+            "forward_s": t1 - t0, "adjoint_s": t2 - t1, "adjoint_over_forward": (t2 - t1) / (t1 - t0),
+            "dL_dvd": g_vd, "dL_dvd_fd": fd_vd, "dL_dE0": g_e0, "dL_dE0_fd": fd_e0,
+            "rel_err_vd": abs(g_vd - fd_vd) / abs(fd_vd), "rel_err_E0": abs(g_e0 - fd_e0) / abs(fd_e0),
+            "precision": "FP64 particle path, FP64 torch grid operators",
+            "note": "adjoint time includes recomputing every window's states"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,6 +178,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-steps", type=int, default=50)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the particle path runs on one MI355X that owns the "
@@ -210,6 +245,13 @@ def main():
         log(f"[bench] fp64: {el64 / steps64 * 1e3:.3f} ms/step; fp32/fp64 = {v32 / v64:.2f}x; "
             f"density err {derr}")
 
+    adj = None
+    if not args.no_adjoint:
+        adj = adjoint_window(args.adjoint_steps, 10)
+        log(f"[bench] adjoint window: forward {adj['forward_s']:.2f} s, adjoint {adj['adjoint_s']:.2f} s "
+            f"(x{adj['adjoint_over_forward']:.2f}); grad rel err {adj['rel_err_vd']:.1e}, "
+            f"{adj['rel_err_E0']:.1e}")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(w, su, eb, args.cpu_sample, args.cpu_steps)
@@ -234,7 +276,8 @@ def main():
                    "form": "FP32 cell index + in-cell offset push, int64 fixed-point CIC charge "
                            f"(2^24 per unit weight), tile {args.tile}^3 cells",
                    "kinetic_energy_fp32": ek32.tolist(),
-                   "fp64": fp64},
+                   "fp64": fp64,
+                   "adjoint": adj},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(w.name, args.tile),

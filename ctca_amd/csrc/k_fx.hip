@@ -56,7 +56,6 @@ __device__ __forceinline__ int pack_cell(int ix, int iy, int iz) {
 struct FxArgs {
   Grid g;
   int ntx, nty, ntz, ntiles, nb, nspec;
-  int near_ok;  // at least three tiles per axis: the 26 neighbours are distinct
   StateFx in, out;
   // input buffer
   const int64_t *start_in;
@@ -296,21 +295,6 @@ __device__ __forceinline__ int tile_of_cell(int cell, int ntx, int nty) {
   return ((iz >> kLog2T) * nty + (iy >> kLog2T)) * ntx + (ix >> kLog2T);
 }
 
-// direction code of tile t seen from tile o, or kFar
-__device__ __forceinline__ int dir_code(int t, int o, int ntx, int nty, int ntz, int near_ok) {
-  if (!near_ok) return kFar;
-  int dx = t % ntx - o % ntx, dy = (t / ntx) % nty - (o / ntx) % nty,
-      dz = t / (ntx * nty) - o / (ntx * nty);
-  if (dx == ntx - 1) dx = -1;
-  if (dx == 1 - ntx) dx = 1;
-  if (dy == nty - 1) dy = -1;
-  if (dy == 1 - nty) dy = 1;
-  if (dz == ntz - 1) dz = -1;
-  if (dz == 1 - ntz) dz = 1;
-  if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1) return kFar;
-  return (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
-}
-
 // the bucket of the same species whose tile lies in direction `code` of b's tile
 __device__ __forceinline__ int neighbour_bucket(int b, int code, int ntx, int nty, int ntz,
                                                 int ntiles) {
@@ -351,13 +335,27 @@ struct Zig {
   float md1, vv1, md2, vv2;
 };
 
-__device__ __forceinline__ Zig zig_axis(float fb, int d, float fa, float rustep) {
+// A particle that leaves the box through a face takes that face as its relay
+// point and the ghost cell -1 or n as its second cell (psolve1.F90:440-478),
+// whatever the distance moved; d2 returns the second cell relative to ib.
+__device__ __forceinline__ Zig zig_axis(float fb, int d, float fa, int ib, int n, float rustep,
+                                        int &d2) {
   float xa = (float)d + fa;
-  float r = d == 0 ? 0.5f * (fb + xa) : (float)max(0, d);
+  float r;
+  if (ib + d < 0) {
+    r = (float)(-ib);
+    d2 = -1 - ib;
+  } else if (ib + d >= n) {
+    r = (float)(n - ib);
+    d2 = n - ib;
+  } else {
+    r = d == 0 ? 0.5f * (fb + xa) : (float)max(0, d);
+    d2 = d;
+  }
   Zig z;
   z.md1 = 0.5f * (r + fb);
   z.vv1 = (r - fb) * rustep;
-  z.md2 = 0.5f * (xa + r) - (float)d;
+  z.md2 = 0.5f * (xa + r) - (float)d2;
   z.vv2 = (xa - r) * rustep;
   return z;
 }
@@ -437,8 +435,9 @@ __device__ __forceinline__ void zig_seg_global(unsigned long long *j3, int64_t n
 template <int T, bool PUSH, int DEP, bool PID>
 __device__ __forceinline__ void fx_one(const FxArgs &a, int s, bool act, const Pcl &p, int ox,
                                        int oy, int oz, bool lds_ok, uint32_t stay_base,
-                                       int *n_stay, int *n_out, FxRec *outbox, const float4 *fA,
-                                       const float2 *fB, unsigned long long *rh, float ke[3]) {
+                                       int *n_stay, int *n_out, FxRec *outbox,
+                                       unsigned char *ocode, const float4 *fA, const float2 *fB,
+                                       unsigned long long *rh, float ke[3]) {
   constexpr int N1 = T + 1, N3 = T + 3;
   const Grid &g = a.g;
   int ix = p.cell & kCellMask, iy = (p.cell >> kCellBits) & kCellMask,
@@ -525,13 +524,17 @@ __device__ __forceinline__ void fx_one(const FxArgs &a, int s, bool act, const P
     }
   }
   if (DEP == 2) {
-    const Zig zx = zig_axis(fbx, dx, fx, a.rustep), zy = zig_axis(fby, dy, fy, a.rustep),
-              zz = zig_axis(fbz, dz, fz, a.rustep);
-    bool in_halo = in_tile && rmax <= (unsigned)(T + 1);
+    int d2x, d2y, d2z;
+    const Zig zx = zig_axis(fbx, dx, fx, ibx, g.nx, a.rustep, d2x),
+              zy = zig_axis(fby, dy, fy, iby, g.ny, a.rustep, d2y),
+              zz = zig_axis(fbz, dz, fz, ibz, g.nz, a.rustep, d2z);
+    // tile coordinates of the second segment's cell
+    const int sx = lx + d2x + 1, sy = ly + d2y + 1, sz = lz + d2z + 1;
+    bool in_halo = in_tile && max((unsigned)sx, max((unsigned)sy, (unsigned)sz)) <= (unsigned)(T + 1);
     if (act && in_halo) {
       zig_seg_lds<N3>(rh, ((lz + 1) * N3 + (ly + 1)) * N3 + lx + 1, zx.md1, zy.md1, zz.md1,
                       zx.vv1, zy.vv1, zz.vv1);
-      zig_seg_lds<N3>(rh, (rz * N3 + ry) * N3 + rx, zx.md2, zy.md2, zz.md2, zx.vv2, zy.vv2,
+      zig_seg_lds<N3>(rh, (sz * N3 + sy) * N3 + sx, zx.md2, zy.md2, zz.md2, zx.vv2, zy.vv2,
                       zz.vv2);
     }
     if (__ballot(act && !in_halo)) {
@@ -539,8 +542,10 @@ __device__ __forceinline__ void fx_one(const FxArgs &a, int s, bool act, const P
         unsigned long long *j3 = a.jcur + (int64_t)s * 3 * a.ncell;
         zig_seg_global(j3, a.ncell, g, ibx, iby, ibz, zx.md1, zy.md1, zz.md1, zx.vv1, zy.vv1,
                        zz.vv1);
-        zig_seg_global(j3, a.ncell, g, ix, iy, iz, zx.md2, zy.md2, zz.md2, zx.vv2, zy.vv2,
-                       zz.vv2);
+        zig_seg_global(j3, a.ncell, g, wrap_add(ibx, max(-g.nx, min(d2x, g.nx)), g.nx),
+                       wrap_add(iby, max(-g.ny, min(d2y, g.ny)), g.ny),
+                       wrap_add(ibz, max(-g.nz, min(d2z, g.nz)), g.nz), zx.md2, zy.md2, zz.md2,
+                       zx.vv2, zy.vv2, zz.vv2);
       }
     }
   }
@@ -573,6 +578,14 @@ __device__ __forceinline__ void fx_one(const FxArgs &a, int s, bool act, const P
       r.vz = vz;
       r.pid = p.pid;
       outbox[k] = r;
+      // direction of the new tile: the tile coordinate of each axis is 0, 1
+      // or 2 for the tile before, the own tile or the tile after; moves
+      // beyond a neighbouring tile are placed directly at the end
+      ocode[k] = rmax <= (unsigned)(T + 1)
+                     ? (unsigned char)(((rz == 0 ? 0 : (rz <= T ? 1 : 2)) * 3 +
+                                        (ry == 0 ? 0 : (ry <= T ? 1 : 2))) * 3 +
+                                       (rx == 0 ? 0 : (rx <= T ? 1 : 2)))
+                     : (unsigned char)kFar;
     }
     if (__ballot(mover && !boxed)) {  // no outbox (spill rounds) or outbox full
       if (mover && !boxed) {
@@ -620,7 +633,7 @@ __device__ __forceinline__ void pull_table(const FxArgs &a, int b, int64_t *seg_
     const int c = threadIdx.x;
     int n_c = 0;
     int64_t src = 0;
-    if (c < kCodes && c != 13 && a.near_ok) {
+    if (c < kCodes && c != 13) {
       int nbk = neighbour_bucket(b, 26 - c, a.ntx, a.nty, a.ntz, a.ntiles);
       int lo = a.pre_in[(int64_t)nbk * kPre + c], hi = a.pre_in[(int64_t)nbk * kPre + c + 1];
       n_c = hi - lo;
@@ -660,6 +673,7 @@ __global__ __launch_bounds__(kBlock) void k_fx_step(FxArgs a) {
   constexpr int kTileNodes = DEP == 1 ? N3 * N3 * N3 : (DEP == 2 ? 3 * N3 * N3 * N3 : 1);
   __shared__ unsigned long long rh[kTileNodes];  // charge tile, or the three current tiles
   __shared__ FxRec outbox[kOutbox];
+  __shared__ unsigned char ocode[kOutbox];
   __shared__ int64_t seg_src[kCodes];
   __shared__ int seg_pre[kPre];
   __shared__ int n_stay, n_out;
@@ -692,14 +706,14 @@ __global__ __launch_bounds__(kBlock) void k_fx_step(FxArgs a) {
     };
     auto one = [&](bool act, const Pcl &p) {
       fx_one<T, PUSH, DEP, PID>(a, s, act, p, ox, oy, oz, true, stay_base, &n_stay, &n_out,
-                                    outbox, fA, fB, rh, ke);
+                                    outbox, ocode, fA, fB, rh, ke);
     };
     // Software pipelined three deep: while a lane pushes particle k, the loads
     // of particles k + 1 and k + 2 (the next rounds of the bucket) are in
     // flight.  Three state sets rotate through the unrolled body, so no set is
     // copied while its loads are outstanding.  Load indices are clamped to the
     // bucket, so every lane loads.
-    Pcl A, B, C;
+    Pcl A{}, B{}, C{};
     if (cnt > 0) {
       ld(0, A);
       ld(kBlock, B);
@@ -723,32 +737,29 @@ __global__ __launch_bounds__(kBlock) void k_fx_step(FxArgs a) {
     if (DEP != 0)
       for (int q = threadIdx.x; q < kTileNodes; q += kBlock) rh[q] = 0ull;
     __syncthreads();
-    if (cnt > 0) {
-      const int nfull = (cnt / kBlock) * kBlock;
-      int base = 0;
-      for (; base + 3 * kBlock <= nfull; base += 3 * kBlock) {
-        ld(base + 2 * kBlock, C);
-        one(true, A);
-        ld(base + 3 * kBlock, A);
-        one(true, B);
-        ld(base + 4 * kBlock, B);
-        one(true, C);
-      }
-      // the last rounds (fewer than three full ones, then the partial one)
-      for (; base < cnt; base += kBlock) {
-        one(base + (int)threadIdx.x < cnt, A);
-        A = B;
-        ld(base + 2 * kBlock, B);
-      }
+    const int nfull = (cnt / kBlock) * kBlock;
+    int base = 0;
+    for (; base + 3 * kBlock <= nfull; base += 3 * kBlock) {
+      ld(base + 2 * kBlock, C);
+      one(true, A);
+      ld(base + 3 * kBlock, A);
+      one(true, B);
+      ld(base + 4 * kBlock, B);
+      one(true, C);
     }
-    // arrivals pulled from the neighbours' outboxes
-    const int narr = seg_pre[kCodes];
-    for (int j0 = 0; j0 < narr; j0 += kBlock) {
-      const int j = j0 + (int)threadIdx.x;
-      Pcl P;
-      load_rec(a.ob_in, arrival_rec(seg_src, seg_pre, min(j, narr - 1)), P);
-      if (!PID) P.pid = 0;
-      one(j < narr, P);
+    // the last rounds: the remaining own particles, then the arrivals pulled
+    // from the neighbours' outboxes, packed into the same rounds
+    const int total = cnt + seg_pre[kCodes];
+    for (; base < total; base += kBlock) {
+      const int i = base + (int)threadIdx.x;
+      Pcl P = A;
+      if (i >= cnt && i < total) {
+        load_rec(a.ob_in, arrival_rec(seg_src, seg_pre, i - cnt), P);
+        if (!PID) P.pid = 0;
+      }
+      one(i < total, P);
+      A = B;
+      if (cnt > 0) ld(base + 2 * kBlock, B);
     }
     if (DEP != 0) {
       // tile node (rx, ry, rz) of component d -> cell (ox - 1 + rx, ...) of
@@ -784,7 +795,7 @@ __global__ __launch_bounds__(kBlock) void k_fx_step(FxArgs a) {
       Pcl cur{};
       if (i < cnt) load_pcl(a.in, beg + i, cur);
       fx_one<T, PUSH, DEP, PID>(a, s, i < cnt, cur, 0, 0, 0, false, 0u, &n_stay, &n_out,
-                                    outbox, fA, fB, rh, ke);
+                                    outbox, ocode, fA, fB, rh, ke);
     }
   }
   // ---- write the outbox grouped by direction (bucket blocks)
@@ -799,8 +810,7 @@ __global__ __launch_bounds__(kBlock) void k_fx_step(FxArgs a) {
       cq[k] = kFar;
       rq[k] = 0;
       if (q < nout) {
-        cq[k] = dir_code(tile_of_cell<T>(outbox[q].cell, a.ntx, a.nty), t, a.ntx, a.nty, a.ntz,
-                         a.near_ok);
+        cq[k] = ocode[q];
         if (cq[k] < kFar) rq[k] = atomicAdd(&hist[cq[k]], 1);
       }
     }
@@ -964,16 +974,15 @@ __global__ __launch_bounds__(kBlock) void k_fx_energy(FxArgs a) {
 // the neighbours' outboxes and kExtra direct slots; clears the step counters
 __global__ void k_fx_caps(const int32_t *count_in, const int32_t *pre_in, int32_t *cap_out,
                           int32_t *extra, int32_t *spill_out, int nb, int nspec, int ntx, int nty,
-                          int ntz, int ntiles, int near_ok) {
+                          int ntz, int ntiles) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b < nb) {
     int arr = 0;
-    if (near_ok)
-      for (int c = 0; c < kCodes; c++) {
-        if (c == 13) continue;
-        int n = neighbour_bucket(b, 26 - c, ntx, nty, ntz, ntiles);
-        arr += pre_in[(int64_t)n * kPre + c + 1] - pre_in[(int64_t)n * kPre + c];
-      }
+    for (int c = 0; c < kCodes; c++) {
+      if (c == 13) continue;
+      int n = neighbour_bucket(b, 26 - c, ntx, nty, ntz, ntiles);
+      arr += pre_in[(int64_t)n * kPre + c + 1] - pre_in[(int64_t)n * kPre + c];
+    }
     cap_out[b] = count_in[b] + arr + kExtra;
     extra[b] = 0;
   }
@@ -1162,8 +1171,6 @@ __global__ void k_copy_i32(const int32_t *src, int n, int32_t *dst) {
   if (b < n) dst[b] = src[b];
 }
 
-int near_ok(const FxBuckets &B) { return B.ntx >= 3 && B.nty >= 3 && B.ntz >= 3; }
-
 FxArgs make_args(emg_ctx *c) {
   FxBuckets &B = c->fx;
   int in = B.cur, out = 1 - B.cur;
@@ -1175,7 +1182,6 @@ FxArgs make_args(emg_ctx *c) {
   a.ntiles = B.ntiles;
   a.nb = B.nb;
   a.nspec = c->sp.nspec;
-  a.near_ok = near_ok(B);
   a.in = B.st[in];
   a.out = B.st[out];
   a.start_in = B.start[in];
@@ -1310,7 +1316,7 @@ int fx_alloc(emg_ctx *c, int64_t ntotal) {
   B.area = ntotal + (int64_t)kExtra * B.nb;
   int64_t maxs = 0;
   for (int s = 0; s < c->sp.nspec; s++) maxs = c->cfg.capacity[s] > maxs ? c->cfg.capacity[s] : maxs;
-  B.spill_cap = maxs / 64 + 4096;
+  B.spill_cap = maxs / 32 + 4096;
   if (B.spill_cap > (1 << 28)) B.spill_cap = 1 << 28;
   B.cap_total = B.area + B.spill_cap * c->sp.nspec;
   if (B.cap_total >= (int64_t(1) << 29))  // masked stores: byte offsets below 2^31
@@ -1411,7 +1417,7 @@ void launch_fx_step(emg_ctx *c, int ustep, bool push, bool deposit, bool em) {
   unsigned gb = (unsigned)((B.nb + kBlock - 1) / kBlock);
   hipLaunchKernelGGL(k_fx_caps, dim3(gb), dim3(kBlock), 0, c->stream, B.count[in], B.pre[in],
                      B.cap[out], B.extra, B.spill[out], B.nb, c->sp.nspec, B.ntx, B.nty, B.ntz,
-                     B.ntiles, near_ok(B));
+                     B.ntiles);
   exclusive_scan_i32(B.cap[out], B.start[out], B.nb, c->scan_tmp, c->stream);
   if (dep == 1) launch_zero(c->rhofx, sizeof(unsigned long long) * c->ncell * c->sp.nspec, c->stream);
   if (dep == 2)

@@ -210,4 +210,22 @@ def ramp512():
     ], wc=0.2, phiz=90.0, phixy=0.0, ramp_ratio=100.0, seed=50)
 
 
-WORKLOADS = {w().name: w for w in (es32, mag128, em_twostream, adjoint64, ramp512)}
+def ramp512_slab():
+    """One GPU's share of ramp512 split into 8 z-slabs: the densest slab
+    (z in [224, 256), 512x512x32 cells) holds a fraction (100 - 100^(7/8)) / 99
+    = 44.2% of each species (237,339,608 particles each), with the density
+    rising by 100^(1/8) across it.  Macro-particle charge as in the full box
+    (wp scaled by the slab's density), so the kernels see the real shape and
+    occupancy of the worst-loaded GPU."""
+    full = 8 * 512 * 512 * 256
+    frac = (100.0 - 100.0 ** (7.0 / 8.0)) / 99.0
+    npin = int(round(frac * full))
+    wp = float(np.sqrt(frac * 8.0))   # same q per particle as the full box
+    return Workload("ramp512_slab_densest", (512, 512, 32), 0.05, 50, [
+        Species("electron", -1.0, wp, npin, 1.0),
+        Species("ion", 0.01, 0.1 * wp, npin, 0.1),
+    ], wc=0.2, phiz=90.0, phixy=0.0, ramp_ratio=100.0 ** (1.0 / 8.0), seed=51)
+
+
+WORKLOADS = {w().name: w for w in (es32, mag128, em_twostream, adjoint64, ramp512,
+                                   ramp512_slab)}

@@ -353,8 +353,10 @@ def test_push_em_fp64_vs_oracle():
 def test_push_em_fp32_fixed_vs_fp64_oracle(tile, n):
     """FP32 zigzag push: one step against the oracle's psolve1 from the same
     (FP32-rounded) state.  Positions 1e-5 cells, velocities 1e-5 relative, the
-    folded fixed-point current 1e-5 of its maximum.  The grids cover the
-    outbox path (three or more tiles per axis) and the direct path (two)."""
+    folded fixed-point current 1e-5 of its maximum.  The grids have three
+    or more tiles per axis, and two (the two neighbours along such an axis
+    are the same tile).  A few tail particles move two cells in the step, some
+    of them through a box face, which the reference treats by its own rule."""
     counts = (30000, 8000)
     q, qm, rm = (0.002, 0.002), (1.0, 1.0), (0.002, 0.002)
     rng = np.random.default_rng(5)
