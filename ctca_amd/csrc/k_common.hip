@@ -198,14 +198,17 @@ __global__ __launch_bounds__(kBlock) void k_read_current(ReadArgs r, const void 
 
 // ---------------------------------------------------------------- FP64 sort
 // The reference's comparator (int)(a->z - b->z) (psort-struct.c:15-20) orders
-// a species by z slab and leaves the order inside a slab to qsort.  The same
-// order here: a counting sort on the slab index int(z); inside a slab the
-// order is whatever the atomic cursors give.  For the GPU this is also the
-// right order: a slab's gathers hit a two-plane window of the nodal field
-// (L2 resident) while its charge atomics spread over the whole slab.
-__device__ __forceinline__ int64_t slab_key(double z, const Grid &g) {
-  int k = (int)z;
-  return k < 0 ? 0 : (k >= g.nz ? g.nz - 1 : k);
+// a species by z slab and leaves the order inside a slab to qsort.  Here a
+// counting sort on the cell (int(z), int(y), int(x)): z slabs in order, and
+// inside a slab the cells in order, so that the particles of a workgroup
+// gather from and deposit to a small box of nodes (the order inside a cell is
+// whatever the atomic cursors give).
+__device__ __forceinline__ int64_t slab_key(const State64 &s, int64_t m, const Grid &g) {
+  int k = (int)s.z[m], j = (int)s.y[m], i = (int)s.x[m];
+  k = k < 0 ? 0 : (k >= g.nz ? g.nz - 1 : k);
+  j = j < 0 ? 0 : (j >= g.ny ? g.ny - 1 : j);
+  i = i < 0 ? 0 : (i >= g.nx ? g.nx - 1 : i);
+  return ((int64_t)k * g.ny + j) * g.nx + i;
 }
 
 __global__ __launch_bounds__(kBlock) void k_sort_hist(State64 s, SpeciesTab sp, Grid g, int64_t nkey,
@@ -214,7 +217,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_hist(State64 s, SpeciesTab sp, 
   int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (m >= sp.cnt[is]) return;
   m += sp.off[is];
-  atomicAdd(&count[(int64_t)is * nkey + slab_key(s.z[m], g)], 1);
+  atomicAdd(&count[(int64_t)is * nkey + slab_key(s, m, g)], 1);
 }
 
 __global__ __launch_bounds__(kBlock) void k_sort_scatter(State64 s, State64 t, SpeciesTab sp, Grid g,
@@ -224,7 +227,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_scatter(State64 s, State64 t, S
   int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (m >= sp.cnt[is]) return;
   m += sp.off[is];
-  int64_t key = (int64_t)is * nkey + slab_key(s.z[m], g);
+  int64_t key = (int64_t)is * nkey + slab_key(s, m, g);
   int64_t d = sp.off[is] + (start[key] - start[(int64_t)is * nkey]) + atomicAdd(&cursor[key], 1);
   t.x[d] = s.x[m];
   t.y[d] = s.y[m];
@@ -364,7 +367,7 @@ void launch_read_current(emg_ctx *c, double *dst, const int lo[3], const int hi[
 }
 
 int sort64(emg_ctx *c) {
-  int64_t nkey = c->g.nz;
+  int64_t nkey = c->ncell;
   int64_t len = nkey * c->sp.nspec;
   if (c->sort_len < len) {
     if (c->sort_count) (void)hipFree(c->sort_count);

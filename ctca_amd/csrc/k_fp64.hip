@@ -209,28 +209,75 @@ __global__ __launch_bounds__(kBlock) void k_push_es64(State64 s, SpeciesTab sp,
 
 // ---------------------------------------------------------------- charge
 // charge.F90:59-83 + distribute_charge 197-210; rho[is][node mod n] += v_k q(is)
+// charge.F90:152-210 per particle; the particles of a block are a contiguous
+// run of a species (kDepPer per thread).  The sort orders particles by cell
+// and they drift slowly, so most of a block's particles lie in a box around
+// its middle particle's cell: the block sums their weights in an LDS window
+// of that box and adds each non-zero window node to rho with one global
+// atomic; a particle whose eight nodes are not all in the window adds its
+// weights to rho directly.  FP64 sums in any order: within round-off of the
+// sequential sum.
+constexpr int kDepPer = 16;
+constexpr int kWinX = 80, kWinY = 8, kWinZ = 6;  // 30 KB of FP64
+
 __global__ __launch_bounds__(kBlock) void k_deposit64(State64 s, SpeciesTab sp, Grid g,
                                                       double *__restrict__ rho, int64_t ncell,
                                                       int *flag) {
-  int is = blockIdx.y;
-  int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= sp.cnt[is]) return;
-  m += sp.off[is];
-  double x = s.x[m], y = s.y[m], z = s.z[m];
-  domain_guard(x, y, z, g, flag);
-  int i = (int)x, j = (int)y, k = (int)z;
-  Cic w = cic(x, y, z, i, j, k);
-  double qc = sp.q[is];
+  __shared__ double win[kWinX * kWinY * kWinZ];
+  const int is = blockIdx.y;
+  const int64_t beg = (int64_t)blockIdx.x * kBlock * kDepPer;
+  const int64_t cnt = sp.cnt[is];
+  if (beg >= cnt) return;  // uniform over the block
+  const int64_t end = min(beg + (int64_t)kBlock * kDepPer, cnt);
+  const int64_t off = sp.off[is];
+  // window: nodes o .. o + w - 1 per axis (w <= n, so no node appears twice)
+  const int wx = min(kWinX, g.nx), wy = min(kWinY, g.ny), wz = min(kWinZ, g.nz);
+  const int64_t mid = off + min(beg + (int64_t)kBlock * kDepPer / 2, end - 1);
+  const int ox = wrapn(min((int)s.x[mid], g.nx - 1) - wx / 2 + g.nx, g.nx);
+  const int oy = wrapn(min((int)s.y[mid], g.ny - 1) - wy / 2 + g.ny, g.ny);
+  const int oz = wrapn(min((int)s.z[mid], g.nz - 1) - wz / 2 + g.nz, g.nz);
+  for (int q = threadIdx.x; q < wx * wy * wz; q += kBlock) win[q] = 0.0;
+  __syncthreads();
+  const double qc = sp.q[is];
   double *r = rho + (int64_t)is * ncell;
-  int i1 = i + 1, j1 = j + 1, k1 = k + 1;
-  atomicAdd(r + cidx(g, i, j, k1), w.v1 * qc);
-  atomicAdd(r + cidx(g, i1, j, k1), w.v2 * qc);
-  atomicAdd(r + cidx(g, i1, j1, k1), w.v3 * qc);
-  atomicAdd(r + cidx(g, i, j1, k1), w.v4 * qc);
-  atomicAdd(r + cidx(g, i, j, k), w.v5 * qc);
-  atomicAdd(r + cidx(g, i1, j, k), w.v6 * qc);
-  atomicAdd(r + cidx(g, i1, j1, k), w.v7 * qc);
-  atomicAdd(r + cidx(g, i, j1, k), w.v8 * qc);
+  for (int64_t m = beg + threadIdx.x; m < end; m += kBlock) {
+    double x = s.x[off + m], y = s.y[off + m], z = s.z[off + m];
+    domain_guard(x, y, z, g, flag);
+    int i = (int)x, j = (int)y, k = (int)z;
+    Cic w = cic(x, y, z, i, j, k);
+    // window coordinates of the cell (cell n, of a position at L, is cell 0's image)
+    int li = wrapn(i - ox + g.nx, g.nx), lj = wrapn(j - oy + g.ny, g.ny),
+        lk = wrapn(k - oz + g.nz, g.nz);
+    if (li < wx - 1 && lj < wy - 1 && lk < wz - 1) {
+      double *p = win + (lk * wy + lj) * wx + li;
+      atomicAdd(p + wx * wy, w.v1 * qc);
+      atomicAdd(p + wx * wy + 1, w.v2 * qc);
+      atomicAdd(p + wx * wy + wx + 1, w.v3 * qc);
+      atomicAdd(p + wx * wy + wx, w.v4 * qc);
+      atomicAdd(p, w.v5 * qc);
+      atomicAdd(p + 1, w.v6 * qc);
+      atomicAdd(p + wx + 1, w.v7 * qc);
+      atomicAdd(p + wx, w.v8 * qc);
+    } else {
+      int i1 = i + 1, j1 = j + 1, k1 = k + 1;
+      atomicAdd(r + cidx(g, i, j, k1), w.v1 * qc);
+      atomicAdd(r + cidx(g, i1, j, k1), w.v2 * qc);
+      atomicAdd(r + cidx(g, i1, j1, k1), w.v3 * qc);
+      atomicAdd(r + cidx(g, i, j1, k1), w.v4 * qc);
+      atomicAdd(r + cidx(g, i, j, k), w.v5 * qc);
+      atomicAdd(r + cidx(g, i1, j, k), w.v6 * qc);
+      atomicAdd(r + cidx(g, i1, j1, k), w.v7 * qc);
+      atomicAdd(r + cidx(g, i, j1, k), w.v8 * qc);
+    }
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < wx * wy * wz; q += kBlock) {
+    double v = win[q];
+    if (v != 0.0) {
+      int li = q % wx, lj = (q / wx) % wy, lk = q / (wx * wy);
+      atomicAdd(r + cidx(g, ox + li, oy + lj, oz + lk), v);
+    }
+  }
 }
 
 // ---------------------------------------------------------------- psolve1
@@ -387,8 +434,12 @@ void launch_push_es64(emg_ctx *c, int ustep) {
 void launch_deposit64(emg_ctx *c) {
   launch_zero(c->rho64, sizeof(double) * c->ncell * c->sp.nspec, c->stream);
   timer_begin(c, kTDeposit);
-  hipLaunchKernelGGL(k_deposit64, species_grid(c->sp), dim3(kBlock), 0, c->stream, c->s64, c->sp,
-                     c->g, c->rho64, c->ncell, c->dflag);
+  int64_t mx = 1;
+  for (int q = 0; q < c->sp.nspec; q++) mx = c->sp.cnt[q] > mx ? c->sp.cnt[q] : mx;
+  dim3 grid((unsigned)((mx + (int64_t)kBlock * kDepPer - 1) / ((int64_t)kBlock * kDepPer)),
+            c->sp.nspec);
+  hipLaunchKernelGGL(k_deposit64, grid, dim3(kBlock), 0, c->stream, c->s64, c->sp, c->g, c->rho64,
+                     c->ncell, c->dflag);
   timer_end(c, kTDeposit);
 }
 

@@ -157,8 +157,8 @@ int emg_get_current(emg_ctx *ctx, double *aj, const int32_t lo[3], const int32_t
 /*
  * spsort(ps) (src/etc/spsort.F90:4-39 -> lib/psort/psort-struct.c:7-9):
  * reorder each species segment by z slab int(z), the order the reference's
- * comparator (int)(a->z - b->z) produces; the order inside a slab is not
- * specified (the reference leaves it to qsort).  In EMG_FP32_FIXED the
+ * comparator (int)(a->z - b->z) produces; inside a slab (which the reference
+ * leaves to qsort) the particles are ordered by cell.  In EMG_FP32_FIXED the
  * particles are kept in tile buckets by every step; emg_sort there compacts
  * the buckets and absorbs any spill region.
  */
