@@ -597,8 +597,9 @@ __device__ __forceinline__ void fx_one(const FxArgs &a, int s, bool act, const P
   store_masked<PID>(a.out, a.nrec, off, cell, p.pid, fx, fy, fz, vx, vy, vz);
 }
 
+template <int NT>
 __device__ __forceinline__ void block_reduce3(double ke[3], double *out) {
-  __shared__ double red[3][kBlock / 64];
+  __shared__ double red[3][NT / 64];
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
   for (int d = 0; d < 3; d++) {
@@ -610,7 +611,7 @@ __device__ __forceinline__ void block_reduce3(double ke[3], double *out) {
   __syncthreads();
   if (threadIdx.x < 3) {
     double v = 0.0;
-    for (int k = 0; k < kBlock / 64; k++) v += red[threadIdx.x][k];
+    for (int k = 0; k < NT / 64; k++) v += red[threadIdx.x][k];
     out[threadIdx.x] = v;
   }
 }
@@ -665,9 +666,10 @@ __device__ __forceinline__ int64_t arrival_rec(const int64_t *seg_src, const int
 }
 
 template <int T, bool PUSH, int DEP, bool PID>
-__global__ __launch_bounds__(kBlock) void k_fx_step(FxArgs a) {
+__global__ __launch_bounds__(T == 8 ? 512 : 256) void k_fx_step(FxArgs a) {
+  constexpr int NT = T == 8 ? 512 : 256;  // threads: T^3 cells x 64 particles per cell
   constexpr int N1 = T + 1, N3 = T + 3;
-  constexpr int kPer = kOutbox / kBlock;
+  constexpr int kPer = kOutbox / NT;
   __shared__ float4 fA[PUSH ? N1 * N1 * N1 : 1];
   __shared__ float2 fB[PUSH ? N1 * N1 * N1 : 1];
   constexpr int kTileNodes = DEP == 1 ? N3 * N3 * N3 : (DEP == 2 ? 3 * N3 * N3 * N3 : 1);
@@ -716,11 +718,11 @@ __global__ __launch_bounds__(kBlock) void k_fx_step(FxArgs a) {
     Pcl A{}, B{}, C{};
     if (cnt > 0) {
       ld(0, A);
-      ld(kBlock, B);
+      ld(NT, B);
     }
     pull_table(a, b, seg_src, seg_pre);
     if (PUSH) {
-      for (int q = threadIdx.x; q < 2 * N1 * N1 * N1; q += kBlock) {
+      for (int q = threadIdx.x; q < 2 * N1 * N1 * N1; q += NT) {
         int nq = q >> 1;
         int lx = nq % N1, ly = (nq / N1) % N1, lz = nq / (N1 * N1);
         int gx = wrapc(ox + lx, a.g.nx), gy = wrapc(oy + ly, a.g.ny), gz = wrapc(oz + lz, a.g.nz);
@@ -735,22 +737,22 @@ __global__ __launch_bounds__(kBlock) void k_fx_step(FxArgs a) {
       }
     }
     if (DEP != 0)
-      for (int q = threadIdx.x; q < kTileNodes; q += kBlock) rh[q] = 0ull;
+      for (int q = threadIdx.x; q < kTileNodes; q += NT) rh[q] = 0ull;
     __syncthreads();
-    const int nfull = (cnt / kBlock) * kBlock;
+    const int nfull = (cnt / NT) * NT;
     int base = 0;
-    for (; base + 3 * kBlock <= nfull; base += 3 * kBlock) {
-      ld(base + 2 * kBlock, C);
+    for (; base + 3 * NT <= nfull; base += 3 * NT) {
+      ld(base + 2 * NT, C);
       one(true, A);
-      ld(base + 3 * kBlock, A);
+      ld(base + 3 * NT, A);
       one(true, B);
-      ld(base + 4 * kBlock, B);
+      ld(base + 4 * NT, B);
       one(true, C);
     }
     // the last rounds: the remaining own particles, then the arrivals pulled
     // from the neighbours' outboxes, packed into the same rounds
     const int total = cnt + seg_pre[kCodes];
-    for (; base < total; base += kBlock) {
+    for (; base < total; base += NT) {
       const int i = base + (int)threadIdx.x;
       Pcl P = A;
       if (i >= cnt && i < total) {
@@ -759,7 +761,7 @@ __global__ __launch_bounds__(kBlock) void k_fx_step(FxArgs a) {
       }
       one(i < total, P);
       A = B;
-      if (cnt > 0) ld(base + 2 * kBlock, B);
+      if (cnt > 0) ld(base + 2 * NT, B);
     }
     if (DEP != 0) {
       // tile node (rx, ry, rz) of component d -> cell (ox - 1 + rx, ...) of
@@ -767,7 +769,7 @@ __global__ __launch_bounds__(kBlock) void k_fx_step(FxArgs a) {
       __syncthreads();
       unsigned long long *r = DEP == 1 ? a.rho + (int64_t)s * a.ncell
                                        : a.jcur + (int64_t)s * 3 * a.ncell;
-      for (int q = threadIdx.x; q < kTileNodes; q += kBlock) {
+      for (int q = threadIdx.x; q < kTileNodes; q += NT) {
         unsigned long long v = rh[q];
         if (v != 0ull) {
           int d = q / (N3 * N3 * N3), qq = q - d * (N3 * N3 * N3);
@@ -788,10 +790,10 @@ __global__ __launch_bounds__(kBlock) void k_fx_step(FxArgs a) {
     int cnt = a.spill_in[s];
     if (cnt > a.spill_cap) cnt = (int)a.spill_cap;
     int64_t beg = a.area + (int64_t)s * a.spill_cap;
-    int rounds = (cnt + kBlock * kSpillBlocksPerSpecies - 1) / (kBlock * kSpillBlocksPerSpecies);
+    int rounds = (cnt + NT * kSpillBlocksPerSpecies - 1) / (NT * kSpillBlocksPerSpecies);
     __syncthreads();
     for (int r = 0; r < rounds; r++) {
-      int i = (r * kSpillBlocksPerSpecies + sub) * kBlock + threadIdx.x;
+      int i = (r * kSpillBlocksPerSpecies + sub) * NT + threadIdx.x;
       Pcl cur{};
       if (i < cnt) load_pcl(a.in, beg + i, cur);
       fx_one<T, PUSH, DEP, PID>(a, s, i < cnt, cur, 0, 0, 0, false, 0u, &n_stay, &n_out,
@@ -806,7 +808,7 @@ __global__ __launch_bounds__(kBlock) void k_fx_step(FxArgs a) {
     int cq[kPer], rq[kPer];
 #pragma unroll
     for (int k = 0; k < kPer; k++) {
-      const int q = (int)threadIdx.x + k * kBlock;
+      const int q = (int)threadIdx.x + k * NT;
       cq[k] = kFar;
       rq[k] = 0;
       if (q < nout) {
@@ -834,7 +836,7 @@ __global__ __launch_bounds__(kBlock) void k_fx_step(FxArgs a) {
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < kPer; k++) {
-      const int q = (int)threadIdx.x + k * kBlock;
+      const int q = (int)threadIdx.x + k * NT;
       if (q < nout) {
         const FxRec r = outbox[q];
         if (cq[k] < kFar) {
@@ -848,7 +850,7 @@ __global__ __launch_bounds__(kBlock) void k_fx_step(FxArgs a) {
   }
   if (PUSH) {
     double kd[3] = {(double)ke[0], (double)ke[1], (double)ke[2]};
-    block_reduce3(kd, a.ke_part + 3 * (int64_t)part);
+    block_reduce3<NT>(kd, a.ke_part + 3 * (int64_t)part);
   }
 }
 
@@ -967,7 +969,7 @@ __global__ __launch_bounds__(kBlock) void k_fx_energy(FxArgs a) {
       add(p);
     }
   }
-  block_reduce3(ke, a.ke_part + 3 * (int64_t)bid);
+  block_reduce3<kBlock>(ke, a.ke_part + 3 * (int64_t)bid);
 }
 
 // output capacities of a step: the bucket's particles, its arrivals waiting in
@@ -1218,11 +1220,14 @@ FxArgs make_args(emg_ctx *c) {
 template <int T, bool PID>
 void step_tp(emg_ctx *c, const FxArgs &a, bool push, int dep, unsigned grid) {
   if (push && dep == 1)
-    hipLaunchKernelGGL((k_fx_step<T, true, 1, PID>), dim3(grid), dim3(kBlock), 0, c->stream, a);
+    hipLaunchKernelGGL((k_fx_step<T, true, 1, PID>), dim3(grid), dim3(T == 8 ? 512 : 256), 0,
+                       c->stream, a);
   else if (push && dep == 2)
-    hipLaunchKernelGGL((k_fx_step<T, true, 2, PID>), dim3(grid), dim3(kBlock), 0, c->stream, a);
+    hipLaunchKernelGGL((k_fx_step<T, true, 2, PID>), dim3(grid), dim3(T == 8 ? 512 : 256), 0,
+                       c->stream, a);
   else
-    hipLaunchKernelGGL((k_fx_step<T, false, 0, PID>), dim3(grid), dim3(kBlock), 0, c->stream, a);
+    hipLaunchKernelGGL((k_fx_step<T, false, 0, PID>), dim3(grid), dim3(T == 8 ? 512 : 256), 0,
+                       c->stream, a);
 }
 
 template <int T>
@@ -1299,7 +1304,9 @@ RangeDesc bucket_desc(emg_ctx *c) {
 
 int fx_alloc(emg_ctx *c, int64_t ntotal) {
   FxBuckets &B = c->fx;
-  int T = c->cfg.tile > 0 ? c->cfg.tile : 4;
+  int T = c->cfg.tile;
+  if (T <= 0)  // default: 8^3-cell tiles where the grid allows them
+    T = (c->g.nx % 8 == 0 && c->g.ny % 8 == 0 && c->g.nz % 8 == 0) ? 8 : 4;
   if (T != 4 && T != 8) return set_error(EMG_ERR_ARG, "tile must be 4 or 8");
   if (c->g.nx % T || c->g.ny % T || c->g.nz % T)
     return set_error(EMG_ERR_ARG, "grid must be a multiple of the tile edge");

@@ -64,7 +64,8 @@ typedef struct {
   int32_t n[3];        /* cells per axis nx, ny, nz (slx = nx, ...)          */
   int32_t nspec;       /* species count, <= EMG_MAX_SPECIES                  */
   int32_t precision;   /* EMG_PLAIN_FP64 or EMG_FP32_FIXED                   */
-  int32_t tile;        /* FP32_FIXED: tile edge in cells (0 = default 4)     */
+  int32_t tile;        /* FP32_FIXED: tile edge in cells, 4 or 8 (0 = 8 when
+                          every axis is a multiple of 8, else 4)             */
   int32_t track_pid;   /* FP32_FIXED: carry pid through every move (+8 B per
                           particle-step); without it download returns pid 0  */
   int32_t reserved;    /* must be 0                                          */
