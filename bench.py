@@ -14,7 +14,20 @@ A step is one push + deposit pass over every particle:
   * FP64 form (beside it): emg_push_es + emg_deposit_rho in EMG_PLAIN_FP64,
     the reference arithmetic (psolve2 then charge).
 The host field solve is not part of a step (it stays on the host in the
-reference); inputs are resident in HBM before the timed region.
+reference); inputs are resident in HBM before the timed region.  The FP64
+form is sorted once before its steps (spsort at initialisation; the
+reference re-sorts every isort = 500 steps), and the sort's kernel time is
+reported beside it, also amortised over 500 steps.
+
+Further legs, reported inside `config` (each can be switched off):
+  * adjoint: the 64^3 ES window of 50 steps with checkpoints every 10 --
+    forward time, reverse-sweep time and their ratio, gradients against
+    central differences;
+  * em_twostream: the EM two-stream workload (256x256x128, 295,279,002
+    particles), psolve1 push + zigzag current, FP32 form with int64
+    fixed-point J beside the FP64 form;
+  * ramp_densest_slab: the densest 512x512x32 z-slab of the 1G exp(z) ramp
+    (474,679,216 particles), the worst-loaded share of 8 GPUs.
 
 Prints one JSON line.  `roofline` is the fused FP32 kernel: algorithmic bytes
 per launch (56 B per particle + 24 B per field node + 8 B per charge node and
@@ -74,6 +87,9 @@ def time_steps(ctx, form, steps, warmup):
     from ctca_amd import emg
 
     def one():
+        if form == "em":
+            ctx.push_em(2)
+            return
         ctx.push_es(2)
         if form == "fp64":
             ctx.deposit_rho(0)
@@ -92,6 +108,147 @@ def time_steps(ctx, form, steps, warmup):
     stats = {k: ctx.kernel_stats(k) for k in (emg.KERNEL_PUSH, emg.KERNEL_DEPOSIT)}
     ctx.enable_timing(False)
     return t1 - t0, stats
+
+
+def auto_tile(n, tile_arg=0):
+    return tile_arg if tile_arg > 0 else (8 if all(v % 8 == 0 for v in n) else 4)
+
+
+def kernel_ms(stats, which):
+    ms, count = stats[which]
+    return ms / max(count, 1)
+
+
+def timed_sort(ctx):
+    """emg_sort under the kernel timers; returns its kernel time in ms."""
+    from ctca_amd import emg
+    ctx.synchronize()
+    ctx.enable_timing(True)
+    ctx.reset_stats()
+    ctx.sort()
+    ctx.synchronize()
+    ms = kernel_ms({emg.KERNEL_SORT: ctx.kernel_stats(emg.KERNEL_SORT)}, emg.KERNEL_SORT)
+    ctx.enable_timing(False)
+    return ms
+
+
+def em_leg(steps32, steps64, warmup, tile_arg=0):
+    """The EM two-stream workload (256x256x128 periodic cells; 268,435,456
+    background electrons at 32 per cell, vth 1.0, plus 26,843,546 beam
+    electrons drifting +5.0 along x; immobile neutralising ions): a step is
+    the psolve1 push + Umeda zigzag current.  FP32 cell+offset form with
+    int64 fixed-point J beside the plain FP64 form (sorted at initialisation,
+    as esses/emses sort at step 0); E of the initial density, no B0."""
+    from ctca_amd import emg, units
+    w = units.em_twostream()
+    su = units.to_sunits(w)
+    n = w.n
+    nsp = len(w.species)
+    counts = su.npin
+    npart = int(counts.sum())
+    ncell = n[0] * n[1] * n[2]
+    spec = units.device_species(w, su)
+    tile = auto_tile(n, tile_arg)
+    ctx = emg.Context(n, su.q, su.qm, su.rm, counts, precision=emg.FP32_FIXED, e0=su.e0,
+                      b0=su.b0, tile=tile)
+    try:
+        ctx.init_device(spec, w.seed)
+        eb = build_fields(ctx, su, n)
+        ctx.set_fields(eb)
+        ctx.push_em(2)
+        j32 = ctx.current()
+        el32, st32 = time_steps(ctx, "em", steps32, warmup)
+    finally:
+        ctx.close()
+    k32 = kernel_ms(st32, emg.KERNEL_PUSH)
+    bytes32 = 56.0 * npart + 24.0 * ncell + 24.0 * ncell * nsp
+    out = {"workload": w.name, "grid": list(n), "particles": npart, "tile": tile,
+           "fp32": {"particle_steps_per_s": npart * steps32 / el32,
+                    "ms_per_step": el32 / steps32 * 1e3, "steps": steps32, "kernel_ms": k32,
+                    "bytes_per_launch": bytes32,
+                    "gbs_algorithmic": bytes32 / (k32 * 1e-3) / 1e9,
+                    "hbm_frac": bytes32 / (k32 * 1e-3) / 1e9 / HBM_PEAK_GBS}}
+    if steps64:
+        ctx = emg.Context(n, su.q, su.qm, su.rm, counts, precision=emg.PLAIN_FP64, e0=su.e0,
+                          b0=su.b0)
+        try:
+            ctx.init_device(spec, w.seed)
+            sort_ms = timed_sort(ctx)
+            ctx.set_fields(eb)
+            ctx.push_em(2)
+            j64 = ctx.current()
+            el64, st64 = time_steps(ctx, "em", steps64, warmup)
+        finally:
+            ctx.close()
+        k64 = kernel_ms(st64, emg.KERNEL_PUSH)
+        bytes64 = 96.0 * npart + 48.0 * (n[0] + 1) * (n[1] + 1) * (n[2] + 1) + 24.0 * ncell * nsp
+        v64 = npart * steps64 / el64
+        out["fp64"] = {"particle_steps_per_s": v64, "ms_per_step": el64 / steps64 * 1e3,
+                       "steps": steps64, "kernel_ms": k64,
+                       "gbs_algorithmic": bytes64 / (k64 * 1e-3) / 1e9,
+                       "sort_ms": sort_ms}
+        out["fp32_over_fp64"] = out["fp32"]["particle_steps_per_s"] / v64
+        # the zigzag current is discontinuous in the end point where the relay
+        # point switches from a midpoint to a cell face, so the few particles
+        # whose FP32 and FP64 end points straddle a face set the maximum; the
+        # RMS shows the arithmetic
+        d = j32 - j64
+        out["current_err_fp32_vs_fp64_one_step"] = {
+            "max_over_max": float(np.abs(d).max() / np.abs(j64).max()),
+            "rms_over_rms": float(np.sqrt((d * d).mean() / (j64 * j64).mean())),
+            "nodes_above_1e-5_of_max": int((np.abs(d) > 1e-5 * np.abs(j64).max()).sum()),
+            "nodes": int(d.size)}
+    return out
+
+
+def slab_leg(steps32, steps64, warmup, tile_arg=0):
+    """The densest z-slab of the 1G ramp (512x512x32 cells, 474,679,216
+    particles: the share of one of 8 GPUs that carries the most particles):
+    ES push + fixed-point charge in the FP32 form, the FP64 form beside it."""
+    from ctca_amd import emg, units
+    w = units.ramp512_slab()
+    su = units.to_sunits(w)
+    n = w.n
+    nsp = len(w.species)
+    counts = su.npin
+    npart = int(counts.sum())
+    ncell = n[0] * n[1] * n[2]
+    spec = units.device_species(w, su)
+    tile = auto_tile(n, tile_arg)
+    ctx = emg.Context(n, su.q, su.qm, su.rm, counts, precision=emg.FP32_FIXED, e0=su.e0,
+                      b0=su.b0, tile=tile)
+    try:
+        ctx.init_device(spec, w.seed)
+        eb = build_fields(ctx, su, n)
+        ctx.set_fields(eb)
+        ctx.push_es(2)
+        el32, st32 = time_steps(ctx, "fp32", steps32, warmup)
+    finally:
+        ctx.close()
+    k32 = kernel_ms(st32, emg.KERNEL_PUSH)
+    bytes32 = 56.0 * npart + 24.0 * ncell + 8.0 * ncell * nsp
+    out = {"workload": w.name, "grid": list(n), "particles": npart, "tile": tile,
+           "fp32": {"particle_steps_per_s": npart * steps32 / el32,
+                    "ms_per_step": el32 / steps32 * 1e3, "steps": steps32, "kernel_ms": k32,
+                    "gbs_algorithmic": bytes32 / (k32 * 1e-3) / 1e9,
+                    "hbm_frac": bytes32 / (k32 * 1e-3) / 1e9 / HBM_PEAK_GBS}}
+    if steps64:
+        ctx = emg.Context(n, su.q, su.qm, su.rm, counts, precision=emg.PLAIN_FP64, e0=su.e0,
+                          b0=su.b0)
+        try:
+            ctx.init_device(spec, w.seed)
+            sort_ms = timed_sort(ctx)
+            ctx.set_fields(eb)
+            el64, st64 = time_steps(ctx, "fp64", steps64, warmup)
+        finally:
+            ctx.close()
+        v64 = npart * steps64 / el64
+        out["fp64"] = {"particle_steps_per_s": v64, "ms_per_step": el64 / steps64 * 1e3,
+                       "steps": steps64, "push_kernel_ms": kernel_ms(st64, emg.KERNEL_PUSH),
+                       "deposit_kernel_ms": kernel_ms(st64, emg.KERNEL_DEPOSIT),
+                       "sort_ms": sort_ms}
+        out["fp32_over_fp64"] = out["fp32"]["particle_steps_per_s"] / v64
+    return out
 
 
 def cpu_baseline(w, su, eb, nsample, steps):
@@ -155,11 +312,13 @@ def adjoint_window(steps, checkpoint):
         # central differences at three step sizes: the loss is piecewise smooth
         # (a particle crossing a cell face kinks it), so a large step averages
         # over kinks and a tiny one meets round-off; the adjoint is the limit
+        # (steps relative to the parameter, and the absolute h = 1e-3)
+        fd_steps = [(f"rel {r:g}", r * abs(vd), r * abs(e0)) for r in (1e-4, 1e-6, 1e-8)]
+        fd_steps.append(("h 0.001", 1e-3, 1e-3))
         fd = {}
-        for rel in (1e-4, 1e-6, 1e-8):
-            hv, he = rel * abs(vd), rel * abs(e0)
-            fd[rel] = ((win.forward(vd + hv, e0) - win.forward(vd - hv, e0)) / (2 * hv),
-                       (win.forward(vd, e0 + he) - win.forward(vd, e0 - he)) / (2 * he))
+        for label, hv, he in fd_steps:
+            fd[label] = ((win.forward(vd + hv, e0) - win.forward(vd - hv, e0)) / (2 * hv),
+                         (win.forward(vd, e0 + he) - win.forward(vd, e0 - he)) / (2 * he))
     finally:
         win.close()
     # per parameter, the step whose difference quotient lies closest
@@ -170,7 +329,7 @@ def adjoint_window(steps, checkpoint):
             "checkpoint_every": checkpoint, "loss": loss,
             "forward_s": t1 - t0, "adjoint_s": t2 - t1, "adjoint_over_forward": (t2 - t1) / (t1 - t0),
             "dL_dvd": g_vd, "dL_dE0": g_e0,
-            "fd": {f"{r:g}": {"dL_dvd": v[0], "dL_dE0": v[1],
+            "fd": {r: {"dL_dvd": v[0], "dL_dE0": v[1],
                               "rel_err_vd": abs(g_vd - v[0]) / abs(v[0]),
                               "rel_err_E0": abs(g_e0 - v[1]) / abs(v[1])} for r, v in fd.items()},
             "fd_step_closest": {"vd": best_vd, "E0": best_e0},
@@ -188,11 +347,15 @@ def main():
     ap.add_argument("--tile", type=int, default=0, help="4 or 8 (0: 8 where the grid allows)")
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--cpu-sample", type=int, default=16_777_216)
-    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=50)
+    ap.add_argument("--no-em", action="store_true")
+    ap.add_argument("--em-steps", type=int, default=40)
+    ap.add_argument("--no-slab", action="store_true")
+    ap.add_argument("--slab-steps", type=int, default=40)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the particle path runs on one MI355X that owns the "
@@ -209,7 +372,7 @@ def main():
     npart = int(counts.sum())
     ncell = n[0] * n[1] * n[2]
     spec = units.device_species(w, su)
-    tile = args.tile if args.tile > 0 else (8 if all(v % 8 == 0 for v in n) else 4)
+    tile = auto_tile(n, args.tile)
     log(f"[bench] {w.name}: {npart:,} particles, grid {n}")
 
     # ---- FP32 cell + offset, int64 fixed point (headline)
@@ -238,7 +401,7 @@ def main():
         ctx = emg.Context(n, su.q, su.qm, su.rm, counts, precision=emg.PLAIN_FP64, e0=su.e0,
                           b0=su.b0)
         ctx.init_device(spec, w.seed)
-        ctx.sort()   # spsort at initialisation (esses.F90:277-284)
+        sort_ms = timed_sort(ctx)   # spsort at initialisation (esses.F90:277-284)
         ctx.set_fields(eb)
         ctx.push_es(2)
         ctx.deposit_rho(2)
@@ -255,7 +418,9 @@ def main():
                 / (pm / max(pn, 1) * 1e-3) / 1e9,
                 "deposit_gbs_algorithmic": (24.0 * npart + 8.0 * ncell * nsp) / (dm / max(dn, 1) * 1e-3) / 1e9,
                 "fp32_over_fp64": v32 / v64,
-                "density_err_fp32_vs_fp64_one_step": derr}
+                "density_err_fp32_vs_fp64_one_step": derr,
+                "sort_ms": sort_ms,
+                "sort_ms_amortised_isort500": sort_ms / 500.0}
         log(f"[bench] fp64: {el64 / steps64 * 1e3:.3f} ms/step; fp32/fp64 = {v32 / v64:.2f}x; "
             f"density err {derr}")
 
@@ -265,6 +430,20 @@ def main():
         log(f"[bench] adjoint window: forward {adj['forward_s']:.2f} s, adjoint {adj['adjoint_s']:.2f} s "
             f"(x{adj['adjoint_over_forward']:.2f}); grad rel err {adj['rel_err_vd']:.1e}, "
             f"{adj['rel_err_E0']:.1e}")
+
+    em = None
+    if not args.no_em:
+        em = em_leg(args.em_steps, max(args.em_steps // 4, 5) if not args.no_fp64 else 0,
+                    args.warmup, args.tile)
+        log(f"[bench] em: fp32 {em['fp32']['ms_per_step']:.3f} ms/step, kernel "
+            f"{em['fp32']['kernel_ms']:.3f} ms ({em['fp32']['gbs_algorithmic']:.0f} GB/s)")
+
+    slab = None
+    if not args.no_slab:
+        slab = slab_leg(args.slab_steps, max(args.slab_steps // 8, 5) if not args.no_fp64 else 0,
+                        args.warmup, args.tile)
+        log(f"[bench] ramp slab: fp32 {slab['fp32']['ms_per_step']:.3f} ms/step, kernel "
+            f"{slab['fp32']['kernel_ms']:.3f} ms ({slab['fp32']['gbs_algorithmic']:.0f} GB/s)")
 
     cpu = None
     if not args.no_cpu:
@@ -291,7 +470,9 @@ def main():
                            f"(2^24 per unit weight), tile {tile}^3 cells",
                    "kinetic_energy_fp32": ek32.tolist(),
                    "fp64": fp64,
-                   "adjoint": adj},
+                   "adjoint": adj,
+                   "em_twostream": em,
+                   "ramp_densest_slab": slab},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(w.name, tile),

@@ -375,28 +375,48 @@ __device__ __forceinline__ unsigned long long jfix(float v) {
   return (unsigned long long)(long long)(int)rintf(v * (float)kJScale);
 }
 
-// Add the 12 contributions of one segment at cell base q (LDS tile, N3 nodes
-// per axis) or at cell (i, j, k) of the global accumulators.
+__device__ __forceinline__ int jfix32(float v) { return (int)rintf(v * (float)kJScale); }
+
+// Zigzag current of one particle into the block's current tiles: segment 1
+// at tile cell q1, segment 2 at q2.  When both segments lie in one cell (the
+// particle did not change cell) the two fixed-point contributions of each
+// node are added as integers before a single atomic, which leaves the sums
+// unchanged; only the lanes whose particle changed cell issue the second
+// segment's atomics.
 template <int N3>
-__device__ __forceinline__ void zig_seg_lds(unsigned long long *jh, int q, float mx, float my,
-                                            float mz, float vx, float vy, float vz) {
+__device__ __forceinline__ void zig_lds(unsigned long long *jh, int q1, int q2, const Zig &zx,
+                                        const Zig &zy, const Zig &zz) {
   constexpr int P = N3 * N3 * N3;
-  float W[4];
-  zig_w4(my, mz, W);  // Jx over (y, z)
-  atomicAdd(&jh[q], jfix(vx * W[0]));
-  atomicAdd(&jh[q + N3], jfix(vx * W[1]));
-  atomicAdd(&jh[q + N3 * N3], jfix(vx * W[2]));
-  atomicAdd(&jh[q + N3 + N3 * N3], jfix(vx * W[3]));
-  zig_w4(mx, mz, W);  // Jy over (x, z)
-  atomicAdd(&jh[P + q], jfix(vy * W[0]));
-  atomicAdd(&jh[P + q + 1], jfix(vy * W[1]));
-  atomicAdd(&jh[P + q + N3 * N3], jfix(vy * W[2]));
-  atomicAdd(&jh[P + q + 1 + N3 * N3], jfix(vy * W[3]));
-  zig_w4(mx, my, W);  // Jz over (x, y)
-  atomicAdd(&jh[2 * P + q], jfix(vz * W[0]));
-  atomicAdd(&jh[2 * P + q + 1], jfix(vz * W[1]));
-  atomicAdd(&jh[2 * P + q + N3], jfix(vz * W[2]));
-  atomicAdd(&jh[2 * P + q + 1 + N3], jfix(vz * W[3]));
+  const bool same = q1 == q2;
+  // per component c: the flux and the two transverse midpoint offsets of
+  // each segment, and the node offsets of its four weights (zig_w4 order)
+  const float f1[3] = {zx.vv1, zy.vv1, zz.vv1}, f2[3] = {zx.vv2, zy.vv2, zz.vv2};
+  const float u1[3] = {zy.md1, zx.md1, zx.md1}, w1[3] = {zz.md1, zz.md1, zy.md1};
+  const float u2[3] = {zy.md2, zx.md2, zx.md2}, w2[3] = {zz.md2, zz.md2, zy.md2};
+  const int o[3][4] = {{0, N3, N3 * N3, N3 + N3 * N3},
+                       {P, P + 1, P + N3 * N3, P + 1 + N3 * N3},
+                       {2 * P, 2 * P + 1, 2 * P + N3, 2 * P + 1 + N3}};
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    float W[4], U[4];
+    zig_w4(u1[c], w1[c], W);
+    zig_w4(u2[c], w2[c], U);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int v = jfix32(f1[c] * W[k]) + (same ? jfix32(f2[c] * U[k]) : 0);
+      atomicAdd(&jh[q1 + o[c][k]], (unsigned long long)(long long)v);
+    }
+  }
+  if (!same) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      float U[4];
+      zig_w4(u2[c], w2[c], U);
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        atomicAdd(&jh[q2 + o[c][k]], (unsigned long long)(long long)jfix32(f2[c] * U[k]));
+    }
+  }
 }
 
 __device__ __forceinline__ void zig_seg_global(unsigned long long *j3, int64_t ncell,
@@ -500,13 +520,11 @@ __device__ __forceinline__ void fx_one(const FxArgs &a, int s, bool act, const P
     fixed_weights(fx, fy, fz, W);
     bool in_halo = in_tile && rmax <= (unsigned)(T + 1);
     if (act && in_halo) {
-      int q0 = (rz * N3 + ry) * N3 + rx;
+      const int q0 = (rz * N3 + ry) * N3 + rx;
 #pragma unroll
-      for (int q = 0; q < 7; q++) {
-        int a2 = q & 1, b2 = (q >> 1) & 1, c2 = q >> 2;
-        atomicAdd(&rh[q0 + (c2 * N3 + b2) * N3 + a2], (unsigned long long)(unsigned)W[q]);
-      }
-      atomicAdd(&rh[q0 + (N3 + 1) * N3 + 1], (unsigned long long)(long long)W[7]);
+      for (int q = 0; q < 8; q++)
+        atomicAdd(&rh[q0 + ((q >> 2) * N3 + ((q >> 1) & 1)) * N3 + (q & 1)],
+                  (unsigned long long)(long long)W[q]);
     }
     if (__ballot(act && !in_halo)) {
       if (act && !in_halo) {
@@ -532,10 +550,8 @@ __device__ __forceinline__ void fx_one(const FxArgs &a, int s, bool act, const P
     const int sx = lx + d2x + 1, sy = ly + d2y + 1, sz = lz + d2z + 1;
     bool in_halo = in_tile && max((unsigned)sx, max((unsigned)sy, (unsigned)sz)) <= (unsigned)(T + 1);
     if (act && in_halo) {
-      zig_seg_lds<N3>(rh, ((lz + 1) * N3 + (ly + 1)) * N3 + lx + 1, zx.md1, zy.md1, zz.md1,
-                      zx.vv1, zy.vv1, zz.vv1);
-      zig_seg_lds<N3>(rh, (sz * N3 + sy) * N3 + sx, zx.md2, zy.md2, zz.md2, zx.vv2, zy.vv2,
-                      zz.vv2);
+      const int q1 = ((lz + 1) * N3 + (ly + 1)) * N3 + lx + 1, q2 = (sz * N3 + sy) * N3 + sx;
+      zig_lds<N3>(rh, q1, q2, zx, zy, zz);
     }
     if (__ballot(act && !in_halo)) {
       if (act && !in_halo) {
@@ -616,12 +632,21 @@ __device__ __forceinline__ void block_reduce3(double ke[3], double *out) {
   }
 }
 
-// XCD-aware bucket order: consecutive workgroups go round-robin to the 8 XCDs,
-// so give XCD x the contiguous bucket range [x nb/8, (x+1) nb/8) (neighbouring
-// tiles share field nodes and charge halo nodes in that XCD's L2).
-__device__ __forceinline__ int bucket_of_block(int bid, int nb) {
-  if ((nb & 7) != 0) return bid;
-  return (bid & 7) * (nb >> 3) + (bid >> 3);
+// XCD-aware bucket order.  Consecutive workgroups go round-robin to the 8
+// XCDs, so XCD x takes the contiguous range [x nb/8, (x+1) nb/8) of a work
+// order u that enumerates tiles y-slowest, then z, then x, with the species
+// of one tile adjacent.  Each XCD then holds a y-slab of tiles that spans
+// every z and every species: neighbouring tiles (and the species of one
+// tile) share field and charge halo nodes in that XCD's L2, and a density
+// that varies along z, or species of very different sizes, load the XCDs
+// evenly (a species-major or z-major split would leave some XCDs idle).
+__device__ __forceinline__ int bucket_of_block(int bid, int nb, int nspec, int ntx, int ntz,
+                                               int ntiles) {
+  const int u = (nb & 7) ? bid : (bid & 7) * (nb >> 3) + (bid >> 3);
+  const int s = u % nspec, v = u / nspec;
+  const int tx = v % ntx, r = v / ntx;
+  const int tz = r % ntz, ty = r / ntz;
+  return s * ntiles + (tz * (ntiles / (ntx * ntz)) + ty) * ntx + tx;
 }
 
 // Arrivals of bucket b: segment c (direction c as seen from the source) of the
@@ -690,7 +715,7 @@ __global__ __launch_bounds__(T == 8 ? 512 : 256) void k_fx_step(FxArgs a) {
   }
   if (threadIdx.x < kPre) hist[threadIdx.x] = 0;
   if (bid < a.nb) {
-    b = bucket_of_block(bid, a.nb);
+    b = bucket_of_block(bid, a.nb, a.nspec, a.ntx, a.ntz, a.ntiles);
     part = b;
     s = b / a.ntiles;
     const int t = b - s * a.ntiles;
@@ -876,7 +901,7 @@ __global__ __launch_bounds__(kBlock) void k_fx_deposit(FxArgs a) {
     }
   };
   if (bid < a.nb) {
-    const int b = bucket_of_block(bid, a.nb);
+    const int b = bucket_of_block(bid, a.nb, a.nspec, a.ntx, a.ntz, a.ntiles);
     const int s = b / a.ntiles, t = b - s * a.ntiles;
     const int ox = (t % a.ntx) * T, oy = ((t / a.ntx) % a.nty) * T, oz = (t / (a.ntx * a.nty)) * T;
     const int cnt = a.count_in[b];
