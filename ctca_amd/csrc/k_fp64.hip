@@ -213,17 +213,20 @@ __global__ __launch_bounds__(kBlock) void k_push_es64(State64 s, SpeciesTab sp,
 // run of a species (kDepPer per thread).  The sort orders particles by cell
 // and they drift slowly, so most of a block's particles lie in a box around
 // its middle particle's cell: the block sums their weights in an LDS window
-// of that box and adds each non-zero window node to rho with one global
-// atomic; a particle whose eight nodes are not all in the window adds its
-// weights to rho directly.  FP64 sums in any order: within round-off of the
-// sequential sum.
+// of that box, as int64 fixed point at 2^48 per unit weight (at most
+// kBlock kDepPer terms of at most 2^48 per node), and adds each non-zero
+// window node times q(is) to rho with one global FP64 atomic; a particle
+// whose eight nodes are not all in the window adds its weights to rho
+// directly.  The rounding of a weight (2^-49) is below the FP64 round-off
+// of the sequential sum, which the reference's order would give.
 constexpr int kDepPer = 16;
-constexpr int kWinX = 80, kWinY = 8, kWinZ = 6;  // 30 KB of FP64
+constexpr int kWinX = 80, kWinY = 8, kWinZ = 6;  // 30 KB of int64
+constexpr double kRhoFix = 281474976710656.0;   // 2^48 per unit weight
 
 __global__ __launch_bounds__(kBlock) void k_deposit64(State64 s, SpeciesTab sp, Grid g,
                                                       double *__restrict__ rho, int64_t ncell,
                                                       int *flag) {
-  __shared__ double win[kWinX * kWinY * kWinZ];
+  __shared__ unsigned long long win[kWinX * kWinY * kWinZ];
   const int is = blockIdx.y;
   const int64_t beg = (int64_t)blockIdx.x * kBlock * kDepPer;
   const int64_t cnt = sp.cnt[is];
@@ -236,7 +239,7 @@ __global__ __launch_bounds__(kBlock) void k_deposit64(State64 s, SpeciesTab sp, 
   const int ox = wrapn(min((int)s.x[mid], g.nx - 1) - wx / 2 + g.nx, g.nx);
   const int oy = wrapn(min((int)s.y[mid], g.ny - 1) - wy / 2 + g.ny, g.ny);
   const int oz = wrapn(min((int)s.z[mid], g.nz - 1) - wz / 2 + g.nz, g.nz);
-  for (int q = threadIdx.x; q < wx * wy * wz; q += kBlock) win[q] = 0.0;
+  for (int q = threadIdx.x; q < wx * wy * wz; q += kBlock) win[q] = 0ull;
   __syncthreads();
   const double qc = sp.q[is];
   double *r = rho + (int64_t)is * ncell;
@@ -249,15 +252,16 @@ __global__ __launch_bounds__(kBlock) void k_deposit64(State64 s, SpeciesTab sp, 
     int li = wrapn(i - ox + g.nx, g.nx), lj = wrapn(j - oy + g.ny, g.ny),
         lk = wrapn(k - oz + g.nz, g.nz);
     if (li < wx - 1 && lj < wy - 1 && lk < wz - 1) {
-      double *p = win + (lk * wy + lj) * wx + li;
-      atomicAdd(p + wx * wy, w.v1 * qc);
-      atomicAdd(p + wx * wy + 1, w.v2 * qc);
-      atomicAdd(p + wx * wy + wx + 1, w.v3 * qc);
-      atomicAdd(p + wx * wy + wx, w.v4 * qc);
-      atomicAdd(p, w.v5 * qc);
-      atomicAdd(p + 1, w.v6 * qc);
-      atomicAdd(p + wx + 1, w.v7 * qc);
-      atomicAdd(p + wx, w.v8 * qc);
+      unsigned long long *p = win + (lk * wy + lj) * wx + li;
+      auto fx = [](double v) { return (unsigned long long)__double2ll_rn(v * kRhoFix); };
+      atomicAdd(p + wx * wy, fx(w.v1));
+      atomicAdd(p + wx * wy + 1, fx(w.v2));
+      atomicAdd(p + wx * wy + wx + 1, fx(w.v3));
+      atomicAdd(p + wx * wy + wx, fx(w.v4));
+      atomicAdd(p, fx(w.v5));
+      atomicAdd(p + 1, fx(w.v6));
+      atomicAdd(p + wx + 1, fx(w.v7));
+      atomicAdd(p + wx, fx(w.v8));
     } else {
       int i1 = i + 1, j1 = j + 1, k1 = k + 1;
       atomicAdd(r + cidx(g, i, j, k1), w.v1 * qc);
@@ -272,10 +276,10 @@ __global__ __launch_bounds__(kBlock) void k_deposit64(State64 s, SpeciesTab sp, 
   }
   __syncthreads();
   for (int q = threadIdx.x; q < wx * wy * wz; q += kBlock) {
-    double v = win[q];
-    if (v != 0.0) {
+    const long long v = (long long)win[q];
+    if (v != 0) {
       int li = q % wx, lj = (q / wx) % wy, lk = q / (wx * wy);
-      atomicAdd(r + cidx(g, ox + li, oy + lj, oz + lk), v);
+      atomicAdd(r + cidx(g, ox + li, oy + lj, oz + lk), (double)v * (qc / kRhoFix));
     }
   }
 }
@@ -314,87 +318,145 @@ __device__ __forceinline__ Relay relay_axis(double xnew, double lb, double cb, i
   return r;
 }
 
+// The zigzag sums of a block's particles go to an LDS window (three
+// components, kEmWinX x kEmWinY x kEmWinZ nodes placed around the block's
+// middle particle: after the cell sort a block's particles share a few rows
+// of cells) as int64 fixed point at 2^48 per unit, and reach the global FP64
+// sums once per window node.  A contribution is |vv w| <= 4 in the window
+// (larger fluxes go straight to the global sums), so a node's window sum of
+// at most 2 kBlock kEmPer terms stays below 2^61; the rounding of each term
+// (2^-49) is far below the FP64 summation round-off of the reference's order.
+constexpr int kEmPer = 4;
+constexpr int kEmWinX = 40, kEmWinY = 6, kEmWinZ = 5;  // 28.8 KB of int64
+constexpr double kEmFix = 281474976710656.0;          // 2^48
+
+__device__ __forceinline__ long long em_fix(double v) { return __double2ll_rn(v * kEmFix); }
+
 __global__ __launch_bounds__(kBlock) void k_push_em64(State64 s, SpeciesTab sp,
                                                       const double *__restrict__ node, Grid g,
                                                       int ustep, double *__restrict__ jacc,
                                                       int64_t ncell, int *flag) {
-  int is = blockIdx.y;
-  int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= sp.cnt[is]) return;
-  m += sp.off[is];
+  constexpr int WN = kEmWinX * kEmWinY * kEmWinZ;
+  __shared__ unsigned long long win[3 * WN];
+  const int is = blockIdx.y;
+  const int64_t beg = (int64_t)blockIdx.x * kBlock * kEmPer;
+  const int64_t cnt = sp.cnt[is];
+  if (beg >= cnt) return;  // uniform over the block
+  const int64_t end = min(beg + (int64_t)kBlock * kEmPer, cnt);
+  const int64_t off = sp.off[is];
+  // window: nodes o .. o + w - 1 per axis (w <= n, so no node appears twice)
+  const int wx = min(kEmWinX, g.nx), wy = min(kEmWinY, g.ny), wz = min(kEmWinZ, g.nz);
+  const int64_t mid = off + min(beg + (int64_t)kBlock * kEmPer / 2, end - 1);
+  const int ox = wrapn(min((int)s.x[mid], g.nx - 1) - wx / 2 + g.nx, g.nx);
+  const int oy = wrapn(min((int)s.y[mid], g.ny - 1) - wy / 2 + g.ny, g.ny);
+  const int oz = wrapn(min((int)s.z[mid], g.nz - 1) - wz / 2 + g.nz, g.nz);
+  for (int q = threadIdx.x; q < 3 * WN; q += kBlock) win[q] = 0ull;
+  __syncthreads();
   double gustep = (double)ustep;
   double rustep = 1.0 / gustep;
   double qmp = sp.qm[is] * gustep * 0.5;
-  double x = s.x[m], y = s.y[m], z = s.z[m];
-  double vx = s.vx[m], vy = s.vy[m], vz = s.vz[m];
-  domain_guard(x, y, z, g, flag);
-  double xlb = x, ylb = y, zlb = z;
-  int ib = (int)floor(xlb), jb = (int)floor(ylb), kb = (int)floor(zlb);  // :170-172
-  double xb = ib, yb = jb, zb = kb;
-  Cic w = cic(xlb, ylb, zlb, ib, jb, kb);
-  double f[6];
-  gather6(node, g, w, f);
-  boris64(vx, vy, vz, f, qmp);
-  double xmove = vx * gustep, ymove = vy * gustep, zmove = vz * gustep;
-  x = x + xmove;
-  y = y + ymove;
-  z = z + zmove;
-  bool inside = !(x < 0.0 || x >= g.slx || y < 0.0 || y >= g.sly || z < 0.0 || z >= g.slz);
-  double px, py, pz;
-  Relay rx = relay_axis(x, xlb, xb, ib, g.nx, g.slx, inside, px);
-  Relay ry = relay_axis(y, ylb, yb, jb, g.ny, g.sly, inside, py);
-  Relay rz = relay_axis(z, zlb, zb, kb, g.nz, g.slz, inside, pz);
-  if (!(px >= 0.0 && px <= g.slx && py >= 0.0 && py <= g.sly && pz >= 0.0 && pz <= g.slz))
-    atomicOr(flag, kFlagFar);
-  s.x[m] = px;
-  s.y[m] = py;
-  s.z[m] = pz;
-  s.vx[m] = vx;
-  s.vy[m] = vy;
-  s.vz[m] = vz;
-  // :717-757
-  double xd1 = (rx.r + xlb) * 0.5 - xb, yd1 = (ry.r + ylb) * 0.5 - yb, zd1 = (rz.r + zlb) * 0.5 - zb;
-  double vvx1 = (rx.r - xlb) * rustep, vvy1 = (ry.r - ylb) * rustep, vvz1 = (rz.r - zlb) * rustep;
-  double vx1w4 = yd1 * zd1, vx1w2 = yd1 - vx1w4, vx1w3 = zd1 - vx1w4, vx1w1 = 1.0 - yd1 - vx1w3;
-  double vy1w4 = xd1 * zd1, vy1w2 = xd1 - vy1w4, vy1w3 = zd1 - vy1w4, vy1w1 = 1.0 - xd1 - vy1w3;
-  double vz1w4 = xd1 * yd1, vz1w2 = xd1 - vz1w4, vz1w3 = yd1 - vz1w4, vz1w1 = 1.0 - xd1 - vz1w3;
-  double xd2 = (rx.la + rx.r) * 0.5 - rx.a, yd2 = (ry.la + ry.r) * 0.5 - ry.a,
-         zd2 = (rz.la + rz.r) * 0.5 - rz.a;
-  double vvx2 = (rx.la - rx.r) * rustep, vvy2 = (ry.la - ry.r) * rustep,
-         vvz2 = (rz.la - rz.r) * rustep;
-  double vx2w4 = yd2 * zd2, vx2w2 = yd2 - vx2w4, vx2w3 = zd2 - vx2w4, vx2w1 = 1.0 - yd2 - vx2w3;
-  double vy2w4 = xd2 * zd2, vy2w2 = xd2 - vy2w4, vy2w3 = zd2 - vy2w4, vy2w1 = 1.0 - xd2 - vy2w3;
-  double vz2w4 = xd2 * yd2, vz2w2 = xd2 - vz2w4, vz2w3 = yd2 - vz2w4, vz2w1 = 1.0 - xd2 - vz2w3;
   double *jx = jacc + ((int64_t)is * 3 + 0) * ncell;
   double *jy = jacc + ((int64_t)is * 3 + 1) * ncell;
   double *jz = jacc + ((int64_t)is * 3 + 2) * ncell;
-  int i1b = ib + 1, j1b = jb + 1, k1b = kb + 1;
-  int ia = rx.ia, ja = ry.ia, ka = rz.ia, i1a = ia + 1, j1a = ja + 1, k1a = ka + 1;
-  // :759-789
-  atomicAdd(jx + cidx(g, ib, jb, kb), vvx1 * vx1w1);
-  atomicAdd(jy + cidx(g, ib, jb, kb), vvy1 * vy1w1);
-  atomicAdd(jz + cidx(g, ib, jb, kb), vvz1 * vz1w1);
-  atomicAdd(jx + cidx(g, ib, j1b, kb), vvx1 * vx1w2);
-  atomicAdd(jy + cidx(g, i1b, jb, kb), vvy1 * vy1w2);
-  atomicAdd(jz + cidx(g, i1b, jb, kb), vvz1 * vz1w2);
-  atomicAdd(jx + cidx(g, ib, jb, k1b), vvx1 * vx1w3);
-  atomicAdd(jy + cidx(g, ib, jb, k1b), vvy1 * vy1w3);
-  atomicAdd(jz + cidx(g, ib, j1b, kb), vvz1 * vz1w3);
-  atomicAdd(jx + cidx(g, ib, j1b, k1b), vvx1 * vx1w4);
-  atomicAdd(jy + cidx(g, i1b, jb, k1b), vvy1 * vy1w4);
-  atomicAdd(jz + cidx(g, i1b, j1b, kb), vvz1 * vz1w4);
-  atomicAdd(jx + cidx(g, ia, ja, ka), vvx2 * vx2w1);
-  atomicAdd(jy + cidx(g, ia, ja, ka), vvy2 * vy2w1);
-  atomicAdd(jz + cidx(g, ia, ja, ka), vvz2 * vz2w1);
-  atomicAdd(jx + cidx(g, ia, j1a, ka), vvx2 * vx2w2);
-  atomicAdd(jy + cidx(g, i1a, ja, ka), vvy2 * vy2w2);
-  atomicAdd(jz + cidx(g, i1a, ja, ka), vvz2 * vz2w2);
-  atomicAdd(jx + cidx(g, ia, ja, k1a), vvx2 * vx2w3);
-  atomicAdd(jy + cidx(g, ia, ja, k1a), vvy2 * vy2w3);
-  atomicAdd(jz + cidx(g, ia, j1a, ka), vvz2 * vz2w3);
-  atomicAdd(jx + cidx(g, ia, j1a, k1a), vvx2 * vx2w4);
-  atomicAdd(jy + cidx(g, i1a, ja, k1a), vvy2 * vy2w4);
-  atomicAdd(jz + cidx(g, i1a, j1a, ka), vvz2 * vz2w4);
+  for (int64_t mm = beg + threadIdx.x; mm < end; mm += kBlock) {
+    const int64_t m = off + mm;
+    double x = s.x[m], y = s.y[m], z = s.z[m];
+    double vx = s.vx[m], vy = s.vy[m], vz = s.vz[m];
+    domain_guard(x, y, z, g, flag);
+    double xlb = x, ylb = y, zlb = z;
+    int ib = (int)floor(xlb), jb = (int)floor(ylb), kb = (int)floor(zlb);  // :170-172
+    double xb = ib, yb = jb, zb = kb;
+    Cic w = cic(xlb, ylb, zlb, ib, jb, kb);
+    double f[6];
+    gather6(node, g, w, f);
+    boris64(vx, vy, vz, f, qmp);
+    double xmove = vx * gustep, ymove = vy * gustep, zmove = vz * gustep;
+    x = x + xmove;
+    y = y + ymove;
+    z = z + zmove;
+    bool inside = !(x < 0.0 || x >= g.slx || y < 0.0 || y >= g.sly || z < 0.0 || z >= g.slz);
+    double px, py, pz;
+    Relay rx = relay_axis(x, xlb, xb, ib, g.nx, g.slx, inside, px);
+    Relay ry = relay_axis(y, ylb, yb, jb, g.ny, g.sly, inside, py);
+    Relay rz = relay_axis(z, zlb, zb, kb, g.nz, g.slz, inside, pz);
+    if (!(px >= 0.0 && px <= g.slx && py >= 0.0 && py <= g.sly && pz >= 0.0 && pz <= g.slz))
+      atomicOr(flag, kFlagFar);
+    s.x[m] = px;
+    s.y[m] = py;
+    s.z[m] = pz;
+    s.vx[m] = vx;
+    s.vy[m] = vy;
+    s.vz[m] = vz;
+    // :717-757
+    double xd1 = (rx.r + xlb) * 0.5 - xb, yd1 = (ry.r + ylb) * 0.5 - yb, zd1 = (rz.r + zlb) * 0.5 - zb;
+    double vvx1 = (rx.r - xlb) * rustep, vvy1 = (ry.r - ylb) * rustep, vvz1 = (rz.r - zlb) * rustep;
+    double vx1w4 = yd1 * zd1, vx1w2 = yd1 - vx1w4, vx1w3 = zd1 - vx1w4, vx1w1 = 1.0 - yd1 - vx1w3;
+    double vy1w4 = xd1 * zd1, vy1w2 = xd1 - vy1w4, vy1w3 = zd1 - vy1w4, vy1w1 = 1.0 - xd1 - vy1w3;
+    double vz1w4 = xd1 * yd1, vz1w2 = xd1 - vz1w4, vz1w3 = yd1 - vz1w4, vz1w1 = 1.0 - xd1 - vz1w3;
+    double xd2 = (rx.la + rx.r) * 0.5 - rx.a, yd2 = (ry.la + ry.r) * 0.5 - ry.a,
+           zd2 = (rz.la + rz.r) * 0.5 - rz.a;
+    double vvx2 = (rx.la - rx.r) * rustep, vvy2 = (ry.la - ry.r) * rustep,
+           vvz2 = (rz.la - rz.r) * rustep;
+    double vx2w4 = yd2 * zd2, vx2w2 = yd2 - vx2w4, vx2w3 = zd2 - vx2w4, vx2w1 = 1.0 - yd2 - vx2w3;
+    double vy2w4 = xd2 * zd2, vy2w2 = xd2 - vy2w4, vy2w3 = zd2 - vy2w4, vy2w1 = 1.0 - xd2 - vy2w3;
+    double vz2w4 = xd2 * yd2, vz2w2 = xd2 - vz2w4, vz2w3 = yd2 - vz2w4, vz2w1 = 1.0 - xd2 - vz2w3;
+    // the 12 contributions of each segment (:759-789), in the window when its
+    // cell and the next node along every axis are inside
+    auto seg = [&](int i, int j, int k, double vvx, double vvy, double vvz, const double wx4[4],
+                   const double wy4[4], const double wz4[4]) {
+      const int li = wrapn(i - ox, g.nx), lj = wrapn(j - oy, g.ny), lk = wrapn(k - oz, g.nz);
+      const bool in_win = li <= wx - 2 && lj <= wy - 2 && lk <= wz - 2 && fabs(vvx) <= 4.0 &&
+                          fabs(vvy) <= 4.0 && fabs(vvz) <= 4.0;
+      if (in_win) {
+        unsigned long long *p = win + (lk * wy + lj) * wx + li;
+        const int X = 1, Y = wx, Z = wx * wy;
+        atomicAdd(p, (unsigned long long)em_fix(vvx * wx4[0]));
+        atomicAdd(p + Y, (unsigned long long)em_fix(vvx * wx4[1]));
+        atomicAdd(p + Z, (unsigned long long)em_fix(vvx * wx4[2]));
+        atomicAdd(p + Y + Z, (unsigned long long)em_fix(vvx * wx4[3]));
+        p += WN;
+        atomicAdd(p, (unsigned long long)em_fix(vvy * wy4[0]));
+        atomicAdd(p + X, (unsigned long long)em_fix(vvy * wy4[1]));
+        atomicAdd(p + Z, (unsigned long long)em_fix(vvy * wy4[2]));
+        atomicAdd(p + X + Z, (unsigned long long)em_fix(vvy * wy4[3]));
+        p += WN;
+        atomicAdd(p, (unsigned long long)em_fix(vvz * wz4[0]));
+        atomicAdd(p + X, (unsigned long long)em_fix(vvz * wz4[1]));
+        atomicAdd(p + Y, (unsigned long long)em_fix(vvz * wz4[2]));
+        atomicAdd(p + X + Y, (unsigned long long)em_fix(vvz * wz4[3]));
+      } else {
+        const int i1 = i + 1, j1 = j + 1, k1 = k + 1;
+        atomicAdd(jx + cidx(g, i, j, k), vvx * wx4[0]);
+        atomicAdd(jx + cidx(g, i, j1, k), vvx * wx4[1]);
+        atomicAdd(jx + cidx(g, i, j, k1), vvx * wx4[2]);
+        atomicAdd(jx + cidx(g, i, j1, k1), vvx * wx4[3]);
+        atomicAdd(jy + cidx(g, i, j, k), vvy * wy4[0]);
+        atomicAdd(jy + cidx(g, i1, j, k), vvy * wy4[1]);
+        atomicAdd(jy + cidx(g, i, j, k1), vvy * wy4[2]);
+        atomicAdd(jy + cidx(g, i1, j, k1), vvy * wy4[3]);
+        atomicAdd(jz + cidx(g, i, j, k), vvz * wz4[0]);
+        atomicAdd(jz + cidx(g, i1, j, k), vvz * wz4[1]);
+        atomicAdd(jz + cidx(g, i, j1, k), vvz * wz4[2]);
+        atomicAdd(jz + cidx(g, i1, j1, k), vvz * wz4[3]);
+      }
+    };
+    const double ax1[4] = {vx1w1, vx1w2, vx1w3, vx1w4}, ay1[4] = {vy1w1, vy1w2, vy1w3, vy1w4},
+                 az1[4] = {vz1w1, vz1w2, vz1w3, vz1w4};
+    const double ax2[4] = {vx2w1, vx2w2, vx2w3, vx2w4}, ay2[4] = {vy2w1, vy2w2, vy2w3, vy2w4},
+                 az2[4] = {vz2w1, vz2w2, vz2w3, vz2w4};
+    seg(ib, jb, kb, vvx1, vvy1, vvz1, ax1, ay1, az1);
+    seg(rx.ia, ry.ia, rz.ia, vvx2, vvy2, vvz2, ax2, ay2, az2);
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < 3 * WN; q += kBlock) {
+    const long long v = (long long)win[q];
+    if (v != 0) {
+      const int d = q / WN, r = q - d * WN;
+      const int li = r % wx, lj = (r / wx) % wy, lk = r / (wx * wy);
+      double *jd = d == 0 ? jx : (d == 1 ? jy : jz);
+      atomicAdd(jd + cidx(g, ox + li, oy + lj, oz + lk), (double)v * (1.0 / kEmFix));
+    }
+  }
 }
 
 dim3 species_grid(const SpeciesTab &sp) {
@@ -446,8 +508,12 @@ void launch_deposit64(emg_ctx *c) {
 void launch_push_em64(emg_ctx *c, int ustep) {
   launch_zero(c->j64, sizeof(double) * 3 * c->ncell * c->sp.nspec, c->stream);
   timer_begin(c, kTPush);
-  hipLaunchKernelGGL(k_push_em64, species_grid(c->sp), dim3(kBlock), 0, c->stream, c->s64, c->sp,
-                     c->node64, c->g, ustep, c->j64, c->ncell, c->dflag);
+  int64_t mx = 1;
+  for (int q = 0; q < c->sp.nspec; q++) mx = c->sp.cnt[q] > mx ? c->sp.cnt[q] : mx;
+  dim3 grid((unsigned)((mx + (int64_t)kBlock * kEmPer - 1) / ((int64_t)kBlock * kEmPer)),
+            c->sp.nspec);
+  hipLaunchKernelGGL(k_push_em64, grid, dim3(kBlock), 0, c->stream, c->s64, c->sp, c->node64,
+                     c->g, ustep, c->j64, c->ncell, c->dflag);
   timer_end(c, kTPush);
 }
 

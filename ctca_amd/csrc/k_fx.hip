@@ -375,7 +375,9 @@ __device__ __forceinline__ unsigned long long jfix(float v) {
   return (unsigned long long)(long long)(int)rintf(v * (float)kJScale);
 }
 
-__device__ __forceinline__ int jfix32(float v) { return (int)rintf(v * (float)kJScale); }
+// fixed point of a flux already scaled by kJScale times a weight: the same
+// value as jfix(flux * w), since scaling by a power of two is exact
+__device__ __forceinline__ int jfix_scaled(float fs, float w) { return (int)rintf(fs * w); }
 
 // Zigzag current of one particle into the block's current tiles: segment 1
 // at tile cell q1, segment 2 at q2.  When both segments lie in one cell (the
@@ -388,9 +390,12 @@ __device__ __forceinline__ void zig_lds(unsigned long long *jh, int q1, int q2, 
                                         const Zig &zy, const Zig &zz) {
   constexpr int P = N3 * N3 * N3;
   const bool same = q1 == q2;
-  // per component c: the flux and the two transverse midpoint offsets of
-  // each segment, and the node offsets of its four weights (zig_w4 order)
-  const float f1[3] = {zx.vv1, zy.vv1, zz.vv1}, f2[3] = {zx.vv2, zy.vv2, zz.vv2};
+  // per component c: the flux (times kJScale) and the two transverse midpoint
+  // offsets of each segment, and the node offsets of its four weights
+  // (zig_w4 order)
+  constexpr float S = (float)kJScale;
+  const float f1[3] = {zx.vv1 * S, zy.vv1 * S, zz.vv1 * S},
+              f2[3] = {zx.vv2 * S, zy.vv2 * S, zz.vv2 * S};
   const float u1[3] = {zy.md1, zx.md1, zx.md1}, w1[3] = {zz.md1, zz.md1, zy.md1};
   const float u2[3] = {zy.md2, zx.md2, zx.md2}, w2[3] = {zz.md2, zz.md2, zy.md2};
   const int o[3][4] = {{0, N3, N3 * N3, N3 + N3 * N3},
@@ -403,7 +408,7 @@ __device__ __forceinline__ void zig_lds(unsigned long long *jh, int q1, int q2, 
     zig_w4(u2[c], w2[c], U);
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-      const int v = jfix32(f1[c] * W[k]) + (same ? jfix32(f2[c] * U[k]) : 0);
+      const int v = jfix_scaled(f1[c], W[k]) + (same ? jfix_scaled(f2[c], U[k]) : 0);
       atomicAdd(&jh[q1 + o[c][k]], (unsigned long long)(long long)v);
     }
   }
@@ -414,7 +419,7 @@ __device__ __forceinline__ void zig_lds(unsigned long long *jh, int q1, int q2, 
       zig_w4(u2[c], w2[c], U);
 #pragma unroll
       for (int k = 0; k < 4; k++)
-        atomicAdd(&jh[q2 + o[c][k]], (unsigned long long)(long long)jfix32(f2[c] * U[k]));
+        atomicAdd(&jh[q2 + o[c][k]], (unsigned long long)(long long)jfix_scaled(f2[c], U[k]));
     }
   }
 }
