@@ -743,15 +743,31 @@ __global__ __launch_bounds__(T == 8 ? 512 : 256) void k_fx_step(FxArgs a) {
     auto ld = [&](int i, Pcl &p) {
       load_off<PID>(a.in, off_of(min(i + (int)threadIdx.x, cnt - 1)), p);
     };
+    // particle j of the block's list: the own particles, then the arrivals
+    // pulled from the neighbours' outboxes (needs the pull table); lanes past
+    // the list load nothing
+    int total = 0;
+    auto ldx = [&](int i, Pcl &p) {
+      const int j = i + (int)threadIdx.x;
+      if (j < cnt) {
+        load_off<PID>(a.in, off_of(j), p);
+      } else if (j < total) {
+        load_rec(a.ob_in, arrival_rec(seg_src, seg_pre, j - cnt), p);
+        if (!PID) p.pid = 0;
+      }
+    };
     auto one = [&](bool act, const Pcl &p) {
       fx_one<T, PUSH, DEP, PID>(a, s, act, p, ox, oy, oz, true, stay_base, &n_stay, &n_out,
                                     outbox, ocode, fA, fB, rh, ke);
     };
     // Software pipelined three deep: while a lane pushes particle k, the loads
-    // of particles k + 1 and k + 2 (the next rounds of the bucket) are in
+    // of particles k + 1 and k + 2 (the next rounds of the list) are in
     // flight.  Three state sets rotate through the unrolled body, so no set is
-    // copied while its loads are outstanding.  Load indices are clamped to the
-    // bucket, so every lane loads.
+    // copied while its loads are outstanding.  Rounds of own particles only
+    // load with clamped indices, so every lane loads; the last rounds, which
+    // hold the rest of the own particles and the arrivals, load through the
+    // list.  Lanes of the two rounds loaded ahead that fall on arrivals load
+    // those when the last rounds begin.
     Pcl A{}, B{}, C{};
     if (cnt > 0) {
       ld(0, A);
@@ -776,6 +792,7 @@ __global__ __launch_bounds__(T == 8 ? 512 : 256) void k_fx_step(FxArgs a) {
     if (DEP != 0)
       for (int q = threadIdx.x; q < kTileNodes; q += NT) rh[q] = 0ull;
     __syncthreads();
+    total = cnt + seg_pre[kCodes];
     const int nfull = (cnt / NT) * NT;
     int base = 0;
     for (; base + 3 * NT <= nfull; base += 3 * NT) {
@@ -786,19 +803,16 @@ __global__ __launch_bounds__(T == 8 ? 512 : 256) void k_fx_step(FxArgs a) {
       ld(base + 4 * NT, B);
       one(true, C);
     }
-    // the last rounds: the remaining own particles, then the arrivals pulled
-    // from the neighbours' outboxes, packed into the same rounds
-    const int total = cnt + seg_pre[kCodes];
+    // the last rounds: the remaining own particles, then the arrivals
+    if (base + 2 * NT > cnt) {  // uniform over the block
+      if (base + (int)threadIdx.x >= cnt) ldx(base, A);
+      if (base + NT + (int)threadIdx.x >= cnt) ldx(base + NT, B);
+    }
     for (; base < total; base += NT) {
-      const int i = base + (int)threadIdx.x;
-      Pcl P = A;
-      if (i >= cnt && i < total) {
-        load_rec(a.ob_in, arrival_rec(seg_src, seg_pre, i - cnt), P);
-        if (!PID) P.pid = 0;
-      }
-      one(i < total, P);
+      ldx(base + 2 * NT, C);
+      one(base + (int)threadIdx.x < total, A);
       A = B;
-      if (cnt > 0) ld(base + 2 * NT, B);
+      B = C;
     }
     if (DEP != 0) {
       // tile node (rx, ry, rz) of component d -> cell (ox - 1 + rx, ...) of
