@@ -32,6 +32,9 @@ Further legs, reported inside `config` (each can be switched off):
 Prints one JSON line.  `roofline` is the fused FP32 kernel: algorithmic bytes
 per launch (56 B per particle + 24 B per field node + 8 B per charge node and
 species) over its average launch time from HIP events on the context stream.
+`roofline.copy_gbs` is a plain 1 GiB device copy timed on the same GPU (read +
+write bytes over its time), the practical ceiling of a kernel that streams
+its state once, and `frac_of_copy` the kernel's share of it.
 `roofline.traffic` is the HBM traffic per launch measured with rocprofv3
 FETCH_SIZE / WRITE_SIZE counters on the same workload (profiles/, read here
 when it was measured for this workload and tile; else null).
@@ -72,6 +75,27 @@ def measured_traffic(workload, tile):
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def copy_bandwidth(nbytes=1 << 30, reps=10):
+    """Read + write bandwidth of a plain device-to-device copy of `nbytes` on
+    this GPU (torch copy_, timed with events on its stream): the practical
+    ceiling of a kernel that streams its state in and out once."""
+    import torch
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return 2.0 * nbytes / (ms * 1e-3) / 1e9
 
 
 def build_fields(ctx, su, n):
@@ -251,16 +275,16 @@ def slab_leg(steps32, steps64, warmup, tile_arg=0):
     return out
 
 
-def cpu_baseline(w, su, eb, nsample, steps):
-    """The oracle on host cores: psolve2 + charge + periodic fold, one thread,
-    particles first ordered by the reference's own particle_sort_ (z slabs)."""
+def cpu_sample(w, su, nsample):
+    """A bounded sample of the workload for the oracle: nsample particles split
+    over the species, each species ordered by the reference's own
+    particle_sort_ (z slabs) as spsort leaves it at initialisation."""
     import ctypes as C
 
     import oracle
     n = w.n
     g = oracle.rng_start(w.seed)
     parts = []
-    counts = []
     for s in w.species:
         sp = oracle.SpeciesInit(npin=nsample // len(w.species), peth=s.vth, path=s.vth)
         p = oracle.inipcl_species(g, sp, len(parts) + 1, n, su.t)
@@ -272,7 +296,16 @@ def cpu_baseline(w, su, eb, nsample, steps):
             ref.particle_sort_.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
             ref.particle_sort_(p.ctypes.data_as(C.c_void_p), C.byref(C.c_int(len(p))))
         parts.append(p)
-        counts.append(len(p))
+    return parts
+
+
+def cpu_baseline(w, su, eb, nsample, steps):
+    """The oracle on host cores: psolve2 + charge + periodic fold, one thread
+    (one reference MPI rank), on the z-sorted sample."""
+    import oracle
+    n = w.n
+    parts = cpu_sample(w, su, nsample)
+    counts = [len(p) for p in parts]
     p = np.concatenate(parts)
     wrk = oracle.grdcrt(eb, n, su.e0, su.b0)
     sl = np.array(n, dtype=float)
@@ -288,6 +321,48 @@ def cpu_baseline(w, su, eb, nsample, steps):
             "sample": f"{n[0]}x{n[1]}x{n[2]} grid, {total:,} particles ({len(counts)} species, "
                       f"z-sorted by the reference's particle_sort_), {steps} steps of psolve2 + "
                       f"charge + periodic fold, FP64, 1 thread, {t1 - t0:.1f} s"}
+
+
+def cpu_baseline_threads(w, su, eb, nsample, steps, threads):
+    """The same work spread over host threads: each thread pushes and deposits
+    a contiguous share of every species (a z range, after the sort) into its
+    own charge array; the arrays are summed pairwise in the pool and folded
+    once per step.  The oracle's C calls release the GIL."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import oracle
+    n = w.n
+    parts = cpu_sample(w, su, nsample)
+    shares = []
+    for t in range(threads):
+        sub = [p[len(p) * t // threads: len(p) * (t + 1) // threads] for p in parts]
+        shares.append((np.concatenate(sub), [len(x) for x in sub]))
+    wrk = oracle.grdcrt(eb, n, su.e0, su.b0)
+    sl = np.array(n, dtype=float)
+
+    def work(t):
+        p, cnt = shares[t]
+        oracle.psolve2(p, cnt, su.qm, wrk, n, 2, sl)
+        return oracle.charge(p, cnt, su.q, n)
+
+    def add(ab):
+        np.add(ab[0], ab[1], out=ab[0])
+        return ab[0]
+
+    with ThreadPoolExecutor(threads) as ex:
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            rs = list(ex.map(work, range(threads)))
+            while len(rs) > 1:
+                rs = list(ex.map(add, zip(rs[0::2], rs[1::2]))) + (rs[-1:] if len(rs) % 2 else [])
+            oracle.fold_rho(rs[0], n)
+        t1 = time.perf_counter()
+    total = sum(len(p) for p in parts)
+    return {"value": total * steps / (t1 - t0), "unit": "particle-steps/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{n[0]}x{n[1]}x{n[2]} grid, {total:,} particles, {steps} steps of psolve2 "
+                      f"+ charge + periodic fold, FP64, {threads} threads (per-thread charge "
+                      f"arrays summed each step), {t1 - t0:.1f} s"}
 
 
 def adjoint_window(steps, checkpoint):
@@ -348,6 +423,7 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--cpu-sample", type=int, default=16_777_216)
     ap.add_argument("--cpu-steps", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
@@ -391,6 +467,7 @@ def main():
     kern_ms = push_ms / max(push_n, 1)
     bytes_launch = 56.0 * npart + 24.0 * ncell + 8.0 * ncell * nsp
     achieved = bytes_launch / (kern_ms * 1e-3) / 1e9
+    copy_gbs = copy_bandwidth()
     log(f"[bench] fp32: {el32 / args.steps * 1e3:.3f} ms/step, kernel {kern_ms:.3f} ms, "
         f"{achieved:.0f} GB/s algorithmic")
 
@@ -445,10 +522,15 @@ def main():
         log(f"[bench] ramp slab: fp32 {slab['fp32']['ms_per_step']:.3f} ms/step, kernel "
             f"{slab['fp32']['kernel_ms']:.3f} ms ({slab['fp32']['gbs_algorithmic']:.0f} GB/s)")
 
-    cpu = None
+    cpu = cpu_all = None
     if not args.no_cpu:
         cpu = cpu_baseline(w, su, eb, args.cpu_sample, args.cpu_steps)
         log(f"[bench] cpu baseline {cpu['value']:.3e} particle-steps/s")
+        threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+        # four times the particles per step, so each thread's share outweighs the
+        # per-step work on the whole grid (its charge array, the sum, the fold)
+        cpu_all = cpu_baseline_threads(w, su, eb, 4 * args.cpu_sample, args.cpu_steps, threads)
+        log(f"[bench] cpu baseline, {threads} threads: {cpu_all['value']:.3e} particle-steps/s")
 
     out = {
         "metric": METRIC,
@@ -471,10 +553,12 @@ def main():
                    "kinetic_energy_fp32": ek32.tolist(),
                    "fp64": fp64,
                    "adjoint": adj,
+                   "cpu_all_cores": cpu_all,
                    "em_twostream": em,
                    "ramp_densest_slab": slab},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
+                     "copy_gbs": copy_gbs, "frac_of_copy": achieved / copy_gbs,
                      "traffic": measured_traffic(w.name, tile),
                      "kernel": f"k_fx_step<{tile},true,1> (push + deposit + re-bucket)",
                      "kernel_ms": kern_ms, "bytes_per_launch": bytes_launch},

@@ -270,6 +270,56 @@ def test_fp32_fixed_multistep_with_tile_crossings():
     assert np.allclose(ek32, ok, rtol=1e-5)
 
 
+def test_fp32_ramp_density_with_b_along_x_vs_oracle():
+    """Strongly non-uniform buckets: an exp(z) density ramp of 100 across the
+    box (the 1G configuration's profile) generated on the device, B along x,
+    8^3 tiles.  Three FP32 steps follow the oracle's trajectories to 1e-4 and
+    the per-species density to 1e-5 of its maximum; every particle is kept."""
+    n = (32, 32, 16)
+    counts = (300000, 300000)
+    q, qm, rm = (-0.002, 0.002), (-1.0, 0.01), (0.002, 0.2)
+    t = units.rotation(90.0, 0.0)
+    spec = [dict(count=counts[0], vth_perp=0.1, vth_para=0.1, t=t, ramp_ratio=100.0),
+            dict(count=counts[1], vth_perp=0.01, vth_para=0.01, t=t, ramp_ratio=100.0)]
+    with ctx_for(n, q, qm, rm, counts) as c64:
+        c64.init_device(spec, 77)
+        p, _ = c64.download()
+    p["pid"] = np.arange(len(p))
+    rng = np.random.default_rng(21)
+    eb = periodic_eb(n, rng, 0.02, 0.005)
+    b0 = (0.02, 0.0, 0.0)
+    nsteps = 3
+    with ctx_for(n, q, qm, rm, counts, precision=emg.FP32_FIXED, b0=b0, tile=8,
+                 track_pid=True) as ctx:
+        ctx.upload(p, counts)
+        start, _ = ctx.download()
+        ctx.set_fields(eb)
+        for _ in range(nsteps):
+            ctx.push_es(2)
+        rho32 = [ctx.rho(species=k) for k in range(2)]
+        g, cnt = ctx.download()
+    assert list(cnt) == list(counts)
+    o = by_pid(start)
+    g = by_pid(g)
+    assert np.array_equal(g["pid"], o["pid"])
+    zc = np.histogram(o["z"][:counts[0]], bins=2, range=(0, n[2]))[0]
+    assert zc[1] > 5 * zc[0]   # the buckets differ several-fold in size
+    wrk = oracle.grdcrt(eb, n, (0, 0, 0), b0)
+    for _ in range(nsteps):
+        oracle.psolve2(o, counts, qm, wrk, n, 2, np.array(n, dtype=float))
+    for f, L in zip(("x", "y", "z"), n):
+        d = np.abs(g[f] - o[f])
+        assert np.minimum(d, L - d).max() <= 1e-4, f
+    vt = np.array([0.1, 0.01])[np.repeat([0, 1], counts)]
+    for f in ("vx", "vy", "vz"):
+        assert (np.abs(g[f] - o[f]) / np.maximum(np.abs(o[f]), vt)).max() <= 1e-4, f
+    _, rhodg = oracle.charge(o, counts, q, n, with_species=True)
+    oracle.fold_rho(rhodg, n, ncomp=2)
+    for k in range(2):
+        ok = oracle.rho_nodes(rhodg)[..., k]
+        assert np.abs(rho32[k] - ok).max() <= 1e-5 * np.abs(ok).max(), k
+
+
 def test_fixed_point_rho_is_order_and_tile_independent():
     n = (16, 16, 16)
     counts = (50000, 20000)
