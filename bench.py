@@ -439,6 +439,11 @@ def main():
               "measured mode.", file=sys.stderr)
         sys.exit(2)
 
+    # torch must take the device before libemgpu does: in the other order
+    # torch.cuda.is_available() is False in this process (torch runs the
+    # adjoint leg's grid operators and the copy-bandwidth probe)
+    import torch
+    torch.cuda.is_available()
     from ctca_amd import emg, units
     w = units.WORKLOADS[args.workload]()
     su = units.to_sunits(w)

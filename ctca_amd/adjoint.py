@@ -96,6 +96,8 @@ class ESWindow:
             raise ValueError("steps must be a multiple of the checkpoint interval")
         if self.K // self.C + self.C > emg.EMG_STATE_SLOTS:
             raise ValueError("too many saved states for one context")
+        # (torch has to see the device before libemgpu initialises it in this
+        # process; this call comes before the context is created)
         self.dev = torch.device(device or ("cuda" if torch.cuda.is_available() else "cpu"))
         self.beam = beam
         self.electrons = electrons

@@ -342,6 +342,10 @@ int emg_destroy(emg_ctx *c) {
   dfree(B.extra);
   dfree(B.hist);
   dfree(B.cursor);
+  dfree(B.shist);
+  dfree(B.scur);
+  dfree(B.spre);
+  dfree(B.spl);
   dfree(c->eb_dev);
   dfree(c->node64);
   dfree(c->node32);

@@ -80,6 +80,10 @@ struct FxBuckets {
   int32_t *extra;      // [nb] direct arrivals reserved by atomics (this step)
   FxRec *ob[2];        // [nb * outbox] movers of each bucket, grouped by direction
   int32_t *pre[2];     // [nb * 28] exclusive prefix of the outbox direction counts
+  int32_t *shist;      // [nb] spill particles of the live buffer, by the bucket of their cell
+  int32_t *scur;       // [nb] scatter cursors for them
+  int64_t *spre;       // [nb + 1] exclusive prefix of shist
+  FxRec *spl;          // [nspec * spill_cap] the spill particles sorted by bucket
   StateFx st[2];
   int cur;             // buffer holding the live particles
 };
@@ -170,6 +174,9 @@ void timer_end(emg_ctx *c, int kind);
 // scan: exclusive prefix sum of n int32 values into int64 out[0..n]; tmp >= scan_tmp_need(n)
 int64_t scan_tmp_need(int64_t n);
 void exclusive_scan_i32(const int32_t *in, int64_t *out, int64_t n, int64_t *tmp, hipStream_t s);
+// two scans in one pass (the sum of `in` must stay below 2^32)
+void exclusive_scan2_i32(const int32_t *in, int64_t *out, const int32_t *in2, int64_t *out2,
+                         int64_t n, int64_t *tmp, hipStream_t s);
 
 // kernels: FP64 mirror (k_fp64.hip)
 void launch_relocate64(emg_ctx *c, int elo[3], int eext[3]);

@@ -41,11 +41,18 @@ __device__ int64_t block_excl_scan(int64_t v, int64_t *total) {
   return incl - v;
 }
 
-__global__ __launch_bounds__(kBlock) void k_scan_sums(const int32_t *in, int64_t n, int64_t *bsum) {
+// With a second input in2, each term is in + 2^32 in2: one pass scans both
+// (the first sum must stay below 2^32), and k_scan_final splits the result.
+__device__ __forceinline__ int64_t scan_term(const int32_t *in, const int32_t *in2, int64_t i) {
+  return (int64_t)in[i] + (in2 ? (int64_t)in2[i] * 4294967296ll : 0);
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_sums(const int32_t *in, const int32_t *in2,
+                                                      int64_t n, int64_t *bsum) {
   int64_t base = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * kScanPer;
   int64_t s = 0;
   for (int k = 0; k < kScanPer; k++)
-    if (base + k < n) s += in[base + k];
+    if (base + k < n) s += scan_term(in, in2, base + k);
   int64_t tot;
   block_excl_scan(s, &tot);
   if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
@@ -65,21 +72,29 @@ __global__ __launch_bounds__(kBlock) void k_scan_top(int64_t *bsum, int64_t nblk
   if (threadIdx.x == 0) bsum[nblk] = carry;
 }
 
-__global__ __launch_bounds__(kBlock) void k_scan_final(const int32_t *in, int64_t n,
-                                                       const int64_t *bsum, int64_t nblk,
-                                                       int64_t *out) {
+__global__ __launch_bounds__(kBlock) void k_scan_final(const int32_t *in, const int32_t *in2,
+                                                       int64_t n, const int64_t *bsum,
+                                                       int64_t nblk, int64_t *out, int64_t *out2) {
   int64_t base = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * kScanPer;
   int64_t s = 0;
   for (int k = 0; k < kScanPer; k++)
-    if (base + k < n) s += in[base + k];
+    if (base + k < n) s += scan_term(in, in2, base + k);
   int64_t tot;
   int64_t run = block_excl_scan(s, &tot) + bsum[blockIdx.x];
+  auto put = [&](int64_t i, int64_t v) {
+    if (in2) {
+      out[i] = v & 0xffffffffll;
+      out2[i] = v >> 32;
+    } else {
+      out[i] = v;
+    }
+  };
   for (int k = 0; k < kScanPer; k++)
     if (base + k < n) {
-      out[base + k] = run;
-      run += in[base + k];
+      put(base + k, run);
+      run += scan_term(in, in2, base + k);
     }
-  if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = bsum[nblk];
+  if (blockIdx.x == 0 && threadIdx.x == 0) put(n, bsum[nblk]);
 }
 
 // ---------------------------------------------------------------- energy
@@ -308,15 +323,27 @@ void launch_zero(void *p, size_t bytes, hipStream_t s) {
 
 int64_t scan_tmp_need(int64_t n) { return (n + kScanChunk - 1) / kScanChunk + 1; }
 
-void exclusive_scan_i32(const int32_t *in, int64_t *out, int64_t n, int64_t *tmp, hipStream_t s) {
+static void scan_launch(const int32_t *in, const int32_t *in2, int64_t *out, int64_t *out2,
+                        int64_t n, int64_t *tmp, hipStream_t s) {
   int64_t nblk = (n + kScanChunk - 1) / kScanChunk;
   if (nblk == 0) {
     launch_zero(out, sizeof(int64_t), s);
+    if (out2) launch_zero(out2, sizeof(int64_t), s);
     return;
   }
-  hipLaunchKernelGGL(k_scan_sums, dim3((unsigned)nblk), dim3(kBlock), 0, s, in, n, tmp);
+  hipLaunchKernelGGL(k_scan_sums, dim3((unsigned)nblk), dim3(kBlock), 0, s, in, in2, n, tmp);
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kBlock), 0, s, tmp, nblk);
-  hipLaunchKernelGGL(k_scan_final, dim3((unsigned)nblk), dim3(kBlock), 0, s, in, n, tmp, nblk, out);
+  hipLaunchKernelGGL(k_scan_final, dim3((unsigned)nblk), dim3(kBlock), 0, s, in, in2, n, tmp, nblk,
+                     out, out2);
+}
+
+void exclusive_scan_i32(const int32_t *in, int64_t *out, int64_t n, int64_t *tmp, hipStream_t s) {
+  scan_launch(in, nullptr, out, nullptr, n, tmp, s);
+}
+
+void exclusive_scan2_i32(const int32_t *in, int64_t *out, const int32_t *in2, int64_t *out2,
+                         int64_t n, int64_t *tmp, hipStream_t s) {
+  scan_launch(in, in2, out, out2, n, tmp, s);
 }
 
 void launch_energy64(emg_ctx *c) {

@@ -68,6 +68,9 @@ struct FxArgs {
   const int64_t *start_in;
   const int32_t *count_in, *nst_in, *cap_in, *spill_in, *pre_in;
   const FxRec *ob_in;
+  const FxRec *spl;     // the input buffer's spill particles sorted by bucket
+  const int32_t *shist; // how many of them per bucket
+  const int64_t *spre;  // where each bucket's start
   // output buffer
   const int64_t *start_out;
   const int32_t *cap_out;
@@ -743,16 +746,21 @@ __global__ __launch_bounds__(T == 8 ? 512 : 256) void k_fx_step(FxArgs a) {
     auto ld = [&](int i, Pcl &p) {
       load_off<PID>(a.in, off_of(min(i + (int)threadIdx.x, cnt - 1)), p);
     };
-    // particle j of the block's list: the own particles, then the arrivals
-    // pulled from the neighbours' outboxes (needs the pull table); lanes past
-    // the list load nothing
-    int total = 0;
+    // particle j of the block's list: the own particles, the arrivals pulled
+    // from the neighbours' outboxes (needs the pull table), then the previous
+    // step's spill particles that lie in this tile; lanes past the list load
+    // nothing
+    int total = 0, narr = 0;
+    const int64_t sp0 = a.spre[b];
     auto ldx = [&](int i, Pcl &p) {
       const int j = i + (int)threadIdx.x;
       if (j < cnt) {
         load_off<PID>(a.in, off_of(j), p);
-      } else if (j < total) {
+      } else if (j < cnt + narr) {
         load_rec(a.ob_in, arrival_rec(seg_src, seg_pre, j - cnt), p);
+        if (!PID) p.pid = 0;
+      } else if (j < total) {
+        load_rec(a.spl, sp0 + (j - cnt - narr), p);
         if (!PID) p.pid = 0;
       }
     };
@@ -792,7 +800,8 @@ __global__ __launch_bounds__(T == 8 ? 512 : 256) void k_fx_step(FxArgs a) {
     if (DEP != 0)
       for (int q = threadIdx.x; q < kTileNodes; q += NT) rh[q] = 0ull;
     __syncthreads();
-    total = cnt + seg_pre[kCodes];
+    narr = seg_pre[kCodes];
+    total = cnt + narr + a.shist[b];
     const int nfull = (cnt / NT) * NT;
     int base = 0;
     for (; base + 3 * NT <= nfull; base += 3 * NT) {
@@ -803,7 +812,8 @@ __global__ __launch_bounds__(T == 8 ? 512 : 256) void k_fx_step(FxArgs a) {
       ld(base + 4 * NT, B);
       one(true, C);
     }
-    // the last rounds: the remaining own particles, then the arrivals
+    // the last rounds: the remaining own particles, then the arrivals and the
+    // spill particles
     if (base + 2 * NT > cnt) {  // uniform over the block
       if (base + (int)threadIdx.x >= cnt) ldx(base, A);
       if (base + NT + (int)threadIdx.x >= cnt) ldx(base + NT, B);
@@ -832,26 +842,8 @@ __global__ __launch_bounds__(T == 8 ? 512 : 256) void k_fx_step(FxArgs a) {
         }
       }
     }
-  } else {
-    // spill particles of the previous step: global-memory gather and deposit,
-    // every particle placed directly
-    int k = bid - a.nb;
-    s = k / kSpillBlocksPerSpecies;
-    int sub = k % kSpillBlocksPerSpecies;
-    int cnt = a.spill_in[s];
-    if (cnt > a.spill_cap) cnt = (int)a.spill_cap;
-    int64_t beg = a.area + (int64_t)s * a.spill_cap;
-    int rounds = (cnt + NT * kSpillBlocksPerSpecies - 1) / (NT * kSpillBlocksPerSpecies);
-    __syncthreads();
-    for (int r = 0; r < rounds; r++) {
-      int i = (r * kSpillBlocksPerSpecies + sub) * NT + threadIdx.x;
-      Pcl cur{};
-      if (i < cnt) load_pcl(a.in, beg + i, cur);
-      fx_one<T, PUSH, DEP, PID>(a, s, i < cnt, cur, 0, 0, 0, false, 0u, &n_stay, &n_out,
-                                    outbox, ocode, fA, fB, rh, ke);
-    }
   }
-  // ---- write the outbox grouped by direction (bucket blocks)
+  // ---- write the outbox grouped by direction
   __syncthreads();
   if (b >= 0) {
     const int nout = min(n_out, kOutbox);
@@ -1025,9 +1017,57 @@ __global__ __launch_bounds__(kBlock) void k_fx_energy(FxArgs a) {
 
 // output capacities of a step: the bucket's particles, its arrivals waiting in
 // the neighbours' outboxes and kExtra direct slots; clears the step counters
-__global__ void k_fx_caps(const int32_t *count_in, const int32_t *pre_in, int32_t *cap_out,
-                          int32_t *extra, int32_t *spill_out, int nb, int nspec, int ntx, int nty,
-                          int ntz, int ntiles) {
+// bucket of a stored particle's cell, the cell clamped into the grid
+template <int T>
+__device__ __forceinline__ int bucket_of_cell(int cell, int s, const Grid &g, int ntiles, int ntx,
+                                              int nty) {
+  const int ix = min(cell & kCellMask, g.nx - 1), iy = min((cell >> kCellBits) & kCellMask, g.ny - 1),
+            iz = min((int)((unsigned)cell >> (2 * kCellBits)), g.nz - 1);
+  return s * ntiles + tile_of_cell<T>(pack_cell(ix, iy, iz), ntx, nty);
+}
+
+// The spill particles of the input buffer join the lists of the buckets their
+// cells lie in: count them per bucket (shist is zero on entry) ...
+template <int T>
+__global__ void k_fx_spill_hist(StateFx in, const int32_t *spill, int64_t area, int64_t spill_cap,
+                                Grid g, int ntiles, int ntx, int nty, int32_t *shist) {
+  const int s = blockIdx.y;
+  const int64_t cnt = min((int64_t)spill[s], spill_cap);
+  const int64_t beg = area + (int64_t)s * spill_cap;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt;
+       i += (int64_t)gridDim.x * blockDim.x)
+    atomicAdd(&shist[bucket_of_cell<T>(in.cell[beg + i], s, g, ntiles, ntx, nty)], 1);
+}
+
+// ... and copy them, as outbox records, to their bucket's range of spl
+template <int T>
+__global__ void k_fx_spill_scatter(StateFx in, const int32_t *spill, int64_t area,
+                                   int64_t spill_cap, Grid g, int ntiles, int ntx, int nty,
+                                   const int64_t *spre, int32_t *scur, FxRec *spl) {
+  const int s = blockIdx.y;
+  const int64_t cnt = min((int64_t)spill[s], spill_cap);
+  const int64_t beg = area + (int64_t)s * spill_cap;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    Pcl p;
+    load_pcl(in, beg + i, p);
+    const int b = bucket_of_cell<T>(p.cell, s, g, ntiles, ntx, nty);
+    FxRec r;
+    r.cell = p.cell;
+    r.fx = p.fx;
+    r.fy = p.fy;
+    r.fz = p.fz;
+    r.vx = p.vx;
+    r.vy = p.vy;
+    r.vz = p.vz;
+    r.pid = p.pid;
+    spl[spre[b] + atomicAdd(&scur[b], 1)] = r;
+  }
+}
+
+__global__ void k_fx_caps(const int32_t *count_in, const int32_t *pre_in, const int32_t *shist,
+                          int32_t *cap_out, int32_t *extra, int32_t *spill_out, int nb, int nspec,
+                          int ntx, int nty, int ntz, int ntiles) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b < nb) {
     int arr = 0;
@@ -1036,7 +1076,7 @@ __global__ void k_fx_caps(const int32_t *count_in, const int32_t *pre_in, int32_
       int n = neighbour_bucket(b, 26 - c, ntx, nty, ntz, ntiles);
       arr += pre_in[(int64_t)n * kPre + c + 1] - pre_in[(int64_t)n * kPre + c];
     }
-    cap_out[b] = (int32_t)align_up(count_in[b] + arr + kExtra);
+    cap_out[b] = (int32_t)align_up(count_in[b] + arr + shist[b] + kExtra);
     extra[b] = 0;
   }
   if (b < nspec) spill_out[b] = 0;
@@ -1059,12 +1099,14 @@ __global__ void k_fill_i32(int32_t *p, int64_t n, int v) {
 // to the spill region), spill counters clamped
 __global__ void k_fx_finish(const int32_t *stay, const int32_t *extra, int32_t *nst_out,
                             int32_t *count_out, int nb, int32_t *spill, int nspec,
-                            int32_t spill_cap) {
+                            int32_t spill_cap, int32_t *shist, int32_t *scur) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b < nb) {
     int ne = min(extra[b], kExtra);
     nst_out[b] = stay[b];
     count_out[b] = stay[b] + ne;
+    shist[b] = 0;  // for the next step's spill sort
+    scur[b] = 0;
   }
   if (b < nspec) spill[b] = min(spill[b], spill_cap);
 }
@@ -1244,6 +1286,9 @@ FxArgs make_args(emg_ctx *c) {
   a.spill_in = B.spill[in];
   a.pre_in = B.pre[in];
   a.ob_in = B.ob[in];
+  a.spl = B.spl;
+  a.shist = B.shist;
+  a.spre = B.spre;
   a.start_out = B.start[out];
   a.cap_out = B.cap[out];
   a.stay_out = B.stay;
@@ -1375,7 +1420,7 @@ int fx_alloc(emg_ctx *c, int64_t ntotal) {
   B.area = align_up(ntotal + (int64_t)(kExtra + kAlign - 1) * B.nb);
   int64_t maxs = 0;
   for (int s = 0; s < c->sp.nspec; s++) maxs = c->cfg.capacity[s] > maxs ? c->cfg.capacity[s] : maxs;
-  B.spill_cap = align_up(maxs / 32 + 4096);
+  B.spill_cap = align_up(maxs / 16 + 4096);
   if (B.spill_cap > (1 << 28)) B.spill_cap = 1 << 28;
   B.cap_total = B.area + B.spill_cap * c->sp.nspec;
   if (B.cap_total >= (int64_t(1) << 29))  // masked stores: byte offsets below 2^31
@@ -1417,8 +1462,15 @@ int fx_alloc(emg_ctx *c, int64_t ntotal) {
   if (hipMalloc(&B.stay, sizeof(int32_t) * B.nb) != hipSuccess ||
       hipMalloc(&B.extra, sizeof(int32_t) * B.nb) != hipSuccess ||
       hipMalloc(&B.hist, sizeof(int32_t) * B.nb) != hipSuccess ||
-      hipMalloc(&B.cursor, sizeof(int32_t) * B.nb) != hipSuccess)
+      hipMalloc(&B.cursor, sizeof(int32_t) * B.nb) != hipSuccess ||
+      hipMalloc(&B.shist, sizeof(int32_t) * B.nb) != hipSuccess ||
+      hipMalloc(&B.scur, sizeof(int32_t) * B.nb) != hipSuccess ||
+      hipMalloc(&B.spre, sizeof(int64_t) * (B.nb + 1)) != hipSuccess)
     return set_error(EMG_ERR_NOMEM, "bucket tables");
+  if (hipMalloc(&B.spl, sizeof(FxRec) * B.spill_cap * c->sp.nspec) != hipSuccess)
+    return set_error(EMG_ERR_NOMEM, "sorted spill buffer");
+  launch_zero(B.shist, sizeof(int32_t) * B.nb, c->stream);
+  launch_zero(B.scur, sizeof(int32_t) * B.nb, c->stream);
   int64_t need = scan_tmp_need(B.nb);
   if (need > c->scan_tmp_len) {
     if (c->scan_tmp) (void)hipFree(c->scan_tmp);
@@ -1474,17 +1526,34 @@ void launch_fx_step(emg_ctx *c, int ustep, bool push, bool deposit, bool em) {
   const int dep = em ? 2 : (deposit ? 1 : 0);
   int in = B.cur, out = 1 - B.cur;
   unsigned gb = (unsigned)((B.nb + kBlock - 1) / kBlock);
+  // the input's spill particles, sorted by bucket (the kernels exit at once
+  // when there are none)
+  const dim3 sg(kSpillBlocksPerSpecies, c->sp.nspec);
+  if (B.tile == 4)
+    hipLaunchKernelGGL(k_fx_spill_hist<4>, sg, dim3(kBlock), 0, c->stream, B.st[in], B.spill[in],
+                       B.area, B.spill_cap, c->g, B.ntiles, B.ntx, B.nty, B.shist);
+  else
+    hipLaunchKernelGGL(k_fx_spill_hist<8>, sg, dim3(kBlock), 0, c->stream, B.st[in], B.spill[in],
+                       B.area, B.spill_cap, c->g, B.ntiles, B.ntx, B.nty, B.shist);
   hipLaunchKernelGGL(k_fx_caps, dim3(gb), dim3(kBlock), 0, c->stream, B.count[in], B.pre[in],
-                     B.cap[out], B.extra, B.spill[out], B.nb, c->sp.nspec, B.ntx, B.nty, B.ntz,
-                     B.ntiles);
-  exclusive_scan_i32(B.cap[out], B.start[out], B.nb, c->scan_tmp, c->stream);
+                     B.shist, B.cap[out], B.extra, B.spill[out], B.nb, c->sp.nspec, B.ntx, B.nty,
+                     B.ntz, B.ntiles);
+  exclusive_scan2_i32(B.cap[out], B.start[out], B.shist, B.spre, B.nb, c->scan_tmp, c->stream);
+  if (B.tile == 4)
+    hipLaunchKernelGGL(k_fx_spill_scatter<4>, sg, dim3(kBlock), 0, c->stream, B.st[in],
+                       B.spill[in], B.area, B.spill_cap, c->g, B.ntiles, B.ntx, B.nty, B.spre,
+                       B.scur, B.spl);
+  else
+    hipLaunchKernelGGL(k_fx_spill_scatter<8>, sg, dim3(kBlock), 0, c->stream, B.st[in],
+                       B.spill[in], B.area, B.spill_cap, c->g, B.ntiles, B.ntx, B.nty, B.spre,
+                       B.scur, B.spl);
   if (dep == 1) launch_zero(c->rhofx, sizeof(unsigned long long) * c->ncell * c->sp.nspec, c->stream);
   if (dep == 2)
     launch_zero(c->jfx, sizeof(unsigned long long) * 3 * c->ncell * c->sp.nspec, c->stream);
   FxArgs a = make_args(c);
   a.gstep = (float)ustep;
   a.rustep = ustep > 0 ? 1.f / (float)ustep : 0.f;
-  unsigned grid = (unsigned)(B.nb + c->sp.nspec * kSpillBlocksPerSpecies);
+  unsigned grid = (unsigned)B.nb;
   timer_begin(c, kTPush);
   switch (B.tile) {
     case 4: step_t<4>(c, a, push, dep, grid); break;
@@ -1493,7 +1562,7 @@ void launch_fx_step(emg_ctx *c, int ustep, bool push, bool deposit, bool em) {
   timer_end(c, kTPush);
   hipLaunchKernelGGL(k_fx_finish, dim3(gb), dim3(kBlock), 0, c->stream, B.stay, B.extra,
                      B.nst[out], B.count[out], B.nb, B.spill[out], c->sp.nspec,
-                     (int32_t)B.spill_cap);
+                     (int32_t)B.spill_cap, B.shist, B.scur);
   B.cur = out;
 }
 
@@ -1520,11 +1589,11 @@ void launch_fx_energy(emg_ctx *c) {
   timer_end(c, kTEnergy);
 }
 
-// sum of the fused push partials (spill blocks included)
+// sum of the fused push partials (one per bucket)
 void fx_energy_from_push(emg_ctx *c) {
   FxBuckets &B = c->fx;
   hipLaunchKernelGGL(k_ke_final, dim3(c->sp.nspec), dim3(kBlock), 0, c->stream, c->ke_part,
-                     B.ntiles, B.nb, kSpillBlocksPerSpecies, c->ke_out);
+                     B.ntiles, B.nb, 0, c->ke_out);
 }
 
 }  // namespace emg
