@@ -391,10 +391,10 @@ __device__ __forceinline__ int jfix_scaled(float fs, float w) { return (int)rint
 
 // Zigzag current of one particle into the block's current tiles: segment 1
 // at tile cell q1, segment 2 at q2.  When both segments lie in one cell (the
-// particle did not change cell) the two fixed-point contributions of each
-// node are added as integers before a single atomic, which leaves the sums
-// unchanged; only the lanes whose particle changed cell issue the second
-// segment's atomics.
+// particle did not change cell) the two contributions of each node are summed
+// in FP32 (one fma) and rounded to fixed point once, for a single atomic;
+// only the lanes whose particle changed cell issue the second segment's
+// atomics.
 template <int N3>
 __device__ __forceinline__ void zig_lds(unsigned long long *jh, int q1, int q2, const Zig &zx,
                                         const Zig &zy, const Zig &zz) {
@@ -418,7 +418,7 @@ __device__ __forceinline__ void zig_lds(unsigned long long *jh, int q1, int q2, 
     zig_w4(u2[c], w2[c], U);
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-      const int v = jfix_scaled(f1[c], W[k]) + (same ? jfix_scaled(f2[c], U[k]) : 0);
+      const int v = (int)rintf(fmaf(same ? f2[c] : 0.f, U[k], f1[c] * W[k]));
       atomicAdd(&jh[q1 + o[c][k]], (unsigned long long)(long long)v);
     }
   }
