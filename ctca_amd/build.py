@@ -27,6 +27,7 @@ SOURCES = [
     ("k_common.hip", []),
     ("k_fx.hip", []),
     ("k_adj.hip", []),
+    ("k_yee.hip", ["-ffp-contract=off"]),
     ("emg_api.hip", []),
 ]
 HOST_SOURCES = [("host_init.cpp", ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-I", INC])]

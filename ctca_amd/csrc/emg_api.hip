@@ -347,6 +347,7 @@ int emg_destroy(emg_ctx *c) {
   dfree(B.spre);
   dfree(B.spl);
   dfree(c->eb_dev);
+  dfree(c->fe_buf);
   dfree(c->node64);
   dfree(c->node32);
   dfree(c->rho64);
@@ -569,9 +570,89 @@ int emg_set_fields(emg_ctx *c, const double *eb, const int32_t lo[3], const int3
   if (rc) return rc;
   rc = hip_check(hipMemcpy(c->eb_dev, eb, bytes, hipMemcpyHostToDevice), "eb upload");
   if (rc) return rc;
+  for (int d = 0; d < 3; d++) {
+    c->eb_lo[d] = elo[d];
+    c->eb_ext[d] = ext[d];
+  }
   launch_relocate64(c, elo, ext);
   c->fields_set = true;
+  c->nodes_stale = false;
   return hip_check(hipGetLastError(), "relocate launch");
+}
+
+// ---- device-resident fields: the Yee update on the uploaded eb
+static int need_eb(emg_ctx *c) {
+  if (!c->fields_set || !c->eb_dev) return set_error(EMG_ERR_STATE, "emg_set_fields has not been called");
+  return EMG_OK;
+}
+
+int emg_bfield(emg_ctx *c) {
+  CTX_CHECK(c);
+  int rc = need_eb(c);
+  if (rc) return rc;
+  launch_bfield(c);
+  c->nodes_stale = true;
+  return hip_check(hipGetLastError(), "bfield launch");
+}
+
+int emg_efield(emg_ctx *c, double tcs, double dlt) {
+  CTX_CHECK(c);
+  int rc = need_eb(c);
+  if (rc) return rc;
+  if (!c->j_valid) return set_error(EMG_ERR_STATE, "no current deposited (call emg_push_em first)");
+  launch_efield(c, tcs, dlt);
+  c->nodes_stale = true;
+  return hip_check(hipGetLastError(), "efield launch");
+}
+
+int emg_grdcrt(emg_ctx *c) {
+  CTX_CHECK(c);
+  int rc = need_eb(c);
+  if (rc) return rc;
+  launch_relocate64(c, c->eb_lo, c->eb_ext);
+  c->nodes_stale = false;
+  return hip_check(hipGetLastError(), "relocate launch");
+}
+
+int emg_get_fields(emg_ctx *c, double *eb, const int32_t lo[3], const int32_t hi[3]) {
+  CTX_CHECK(c);
+  int rc = need_eb(c);
+  if (rc) return rc;
+  if (!eb || !lo || !hi) return set_error(EMG_ERR_ARG, "null argument");
+  for (int d = 0; d < 3; d++)
+    if (lo[d] < c->eb_lo[d] || hi[d] > c->eb_lo[d] + c->eb_ext[d] - 1 || hi[d] < lo[d])
+      return set_error(EMG_ERR_ARG, "eb bounds outside the uploaded array");
+  const size_t total = (size_t)6 * c->eb_ext[0] * c->eb_ext[1] * c->eb_ext[2];
+  std::vector<double> all(total);
+  rc = hip_check(hipStreamSynchronize(c->stream), "stream synchronize");
+  if (!rc) rc = hip_check(hipMemcpy(all.data(), c->eb_dev, sizeof(double) * total, hipMemcpyDeviceToHost),
+                          "eb download");
+  if (rc) return rc;
+  const int ex = hi[0] - lo[0] + 1, ey = hi[1] - lo[1] + 1;
+  for (int k = lo[2]; k <= hi[2]; k++)
+    for (int j = lo[1]; j <= hi[1]; j++)
+      for (int i = lo[0]; i <= hi[0]; i++) {
+        const size_t src = ((((size_t)(k - c->eb_lo[2]) * c->eb_ext[1] + (j - c->eb_lo[1])) *
+                                 c->eb_ext[0]) + (i - c->eb_lo[0])) * 6;
+        const size_t dst = ((((size_t)(k - lo[2]) * ey + (j - lo[1])) * ex) + (i - lo[0])) * 6;
+        for (int q = 0; q < 6; q++) eb[dst + q] = all[src + q];
+      }
+  return EMG_OK;
+}
+
+int emg_field_energy(emg_ctx *c, double cs, double *engeb) {
+  CTX_CHECK(c);
+  int rc = need_eb(c);
+  if (rc) return rc;
+  if (!engeb) return set_error(EMG_ERR_ARG, "null argument");
+  if (!c->fe_buf && (rc = dalloc(&c->fe_buf, (size_t)field_energy_part_need() + 2, "field energy")))
+    return rc;
+  launch_field_energy(c, cs, c->fe_buf, c->fe_buf + field_energy_part_need());
+  rc = check_flags(c);
+  if (!rc)
+    rc = hip_check(hipMemcpy(engeb, c->fe_buf + field_energy_part_need(), 2 * sizeof(double),
+                             hipMemcpyDeviceToHost), "field energy copy");
+  return rc;
 }
 
 static int need_particles(emg_ctx *c) {
@@ -581,6 +662,8 @@ static int need_particles(emg_ctx *c) {
 
 static int need_fields(emg_ctx *c) {
   if (!c->fields_set) return set_error(EMG_ERR_STATE, "emg_set_fields has not been called");
+  if (c->nodes_stale)
+    return set_error(EMG_ERR_STATE, "eb changed on the device since the last relocation: call emg_grdcrt");
   return EMG_OK;
 }
 

@@ -117,8 +117,11 @@ struct emg_ctx {
   emg::FxBuckets fx;
 
   // fields
-  double *eb_dev;
+  double *eb_dev;   // eb(6, eb_lo : eb_lo + eb_ext - 1), Fortran layout
   size_t eb_dev_bytes;
+  int eb_lo[3], eb_ext[3];
+  bool nodes_stale;  // eb changed on the device since the last relocation
+  double *fe_buf;    // field-energy partials and result (lazy)
   double *node64;  // (nx+1)(ny+1)(nz+1) x 8 doubles, nodes 0..n
   float *node32;   // nx ny nz x 8 floats, periodic nodes
   bool fields_set;
@@ -184,6 +187,11 @@ void launch_positn64(emg_ctx *c, int ustep);
 void launch_push_es64(emg_ctx *c, int ustep);
 void launch_deposit64(emg_ctx *c);
 void launch_push_em64(emg_ctx *c, int ustep);
+// device Yee update and field energy on eb_dev (k_yee.hip)
+void launch_bfield(emg_ctx *c);
+void launch_efield(emg_ctx *c, double tcs, double dlt);
+int64_t field_energy_part_need();
+void launch_field_energy(emg_ctx *c, double cs, double *part_dev, double *out_dev);
 
 // common kernels (k_common.hip)
 void launch_relocate32(emg_ctx *c);  // node64-independent: from eb_dev

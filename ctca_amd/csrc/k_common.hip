@@ -189,7 +189,9 @@ __global__ __launch_bounds__(kBlock) void k_read_rhofx(ReadArgs r, const unsigne
   out[o] = v;
 }
 
-// aj(3, lo:hi): sum over species of q(is) * J(is) in species order
+// aj(3, lo:hi): sum over species of q(is) * J(is) in species order, each term
+// added with one fused multiply-add (the same arithmetic as the device E
+// update that consumes the current)
 template <bool FIX>
 __global__ __launch_bounds__(kBlock) void k_read_current(ReadArgs r, const void *jacc, double *out) {
   int64_t n = (int64_t)r.ext[0] * r.ext[1] * r.ext[2];
@@ -205,7 +207,7 @@ __global__ __launch_bounds__(kBlock) void k_read_current(ReadArgs r, const void 
       int64_t q = ((int64_t)s * 3 + d) * r.ncell + c;
       double js = FIX ? (double)(long long)((const unsigned long long *)jacc)[q] * (1.0 / kJScale)
                       : ((const double *)jacc)[q];
-      v = v + js * r.q[s];
+      v = fma(js, r.q[s], v);
     }
     out[3 * o + d] = v;
   }

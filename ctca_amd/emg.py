@@ -37,6 +37,7 @@ EXPORTS = (
     "emg_enable_timing", "emg_kernel_stats", "emg_reset_kernel_stats",
     "emg_state_save", "emg_state_load", "emg_adj_reset", "emg_adj_step", "emg_adj_get_field",
     "emg_adj_add_density", "emg_adj_velocity_sums",
+    "emg_bfield", "emg_efield", "emg_grdcrt", "emg_get_fields", "emg_field_energy",
 )
 
 
@@ -111,6 +112,11 @@ def lib():
             "emg_adj_get_field": [vp, vp],
             "emg_adj_add_density": [vp, vp, vp],
             "emg_adj_velocity_sums": [vp, vp],
+            "emg_bfield": [vp],
+            "emg_efield": [vp, C.c_double, C.c_double],
+            "emg_grdcrt": [vp],
+            "emg_get_fields": [vp, vp, i32p, i32p],
+            "emg_field_energy": [vp, C.c_double, vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -252,6 +258,30 @@ class Context:
 
     def push_em(self, ustep):
         _chk(lib().emg_push_em(self.h, int(ustep)))
+
+    # ---- device-resident EM fields (the Yee update on the uploaded eb)
+    def bfield(self):
+        _chk(lib().emg_bfield(self.h))
+
+    def efield(self, tcs, dlt=2.0):
+        _chk(lib().emg_efield(self.h, float(tcs), float(dlt)))
+
+    def grdcrt(self):
+        _chk(lib().emg_grdcrt(self.h))
+
+    def fields(self, lo=(-1, -1, -1), hi=None):
+        """eb(6, lo:hi) as numpy [k, j, i, 6]; default: the box emg_set_fields took
+        for an eb of the usual extent -1..n+1."""
+        if hi is None:
+            hi = tuple(v + 1 for v in self.n)
+        out = np.zeros((hi[2] - lo[2] + 1, hi[1] - lo[1] + 1, hi[0] - lo[0] + 1, 6))
+        _chk(lib().emg_get_fields(self.h, _ptr(out), _i3(lo), _i3(hi)))
+        return out
+
+    def field_energy(self, cs):
+        out = np.zeros(2)
+        _chk(lib().emg_field_energy(self.h, float(cs), _ptr(out)))
+        return out
 
     def sort(self):
         _chk(lib().emg_sort(self.h))

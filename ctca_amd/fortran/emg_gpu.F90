@@ -256,6 +256,38 @@ module emg_gpu
       type(c_ptr), value :: ctx
       real(c_double), dimension(*), intent(out) :: out
     end function emg_adj_velocity_sums
+
+    ! device-resident EM fields: bfield (bfield.F90:38-52), efield
+    ! (efield.F90:55-72), grdcrt on the device copy, read-back, field energy
+    integer(c_int) function emg_bfield(ctx) bind(C, name="emg_bfield")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function emg_bfield
+
+    integer(c_int) function emg_efield(ctx, tcs, dlt) bind(C, name="emg_efield")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: tcs, dlt
+    end function emg_efield
+
+    integer(c_int) function emg_grdcrt(ctx) bind(C, name="emg_grdcrt")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function emg_grdcrt
+
+    integer(c_int) function emg_get_fields(ctx, eb, lo, hi) bind(C, name="emg_get_fields")
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      real(c_double), dimension(*), intent(out) :: eb
+      integer(c_int32_t), dimension(3), intent(in) :: lo, hi
+    end function emg_get_fields
+
+    integer(c_int) function emg_field_energy(ctx, cs, engeb) bind(C, name="emg_field_energy")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: cs
+      real(c_double), dimension(2), intent(out) :: engeb
+    end function emg_field_energy
   end interface
 
 contains

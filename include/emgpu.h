@@ -156,6 +156,30 @@ int emg_push_em(emg_ctx *ctx, int32_t ustep);
 int emg_get_current(emg_ctx *ctx, double *aj, const int32_t lo[3], const int32_t hi[3]);
 
 /*
+ * The EM field update on the library's copy of eb (the array emg_set_fields
+ * uploaded, periodic ghosts kept consistent), so an EM step can stay on the
+ * device: emg_bfield, emg_grdcrt, emg_push_em, emg_bfield, emg_efield.
+ *   emg_bfield  ≙ bfield(ps) (src/field/bfield.F90:38-52): the half-step
+ *               B += (EY(k+1) - EY) - (EZ(j+1) - EZ), etc.
+ *   emg_efield  ≙ efield(ps) (src/field/efield.F90:55-72): E += tcs (curl B
+ *               differences) - dlt aj, aj the folded current of the last
+ *               emg_push_em (tcs = 2 cv^2 in S-units, inital.F90:78 and
+ *               rescal.F90:132; dlt = 2, or 0 for test-particle runs).
+ *   emg_grdcrt  ≙ grdcrt(ps) on the device copy: relocates eb to the nodes
+ *               again (emg_push_es / emg_push_em refuse nodes older than the
+ *               last field update).
+ *   emg_get_fields reads eb(6, lo:hi) back (inside the uploaded bounds).
+ *   emg_field_energy ≙ the field part of energy (src/etc/energy.F90:52-70):
+ *               engeb[0] = 0.5 sum E^2 / V, engeb[1] = 0.5 cs sum B^2 / V over
+ *               the cells 0..n-1.
+ */
+int emg_bfield(emg_ctx *ctx);
+int emg_efield(emg_ctx *ctx, double tcs, double dlt);
+int emg_grdcrt(emg_ctx *ctx);
+int emg_get_fields(emg_ctx *ctx, double *eb, const int32_t lo[3], const int32_t hi[3]);
+int emg_field_energy(emg_ctx *ctx, double cs, double *engeb);
+
+/*
  * spsort(ps) (src/etc/spsort.F90:4-39 -> lib/psort/psort-struct.c:7-9):
  * reorder each species segment by z slab int(z), the order the reference's
  * comparator (int)(a->z - b->z) produces; inside a slab (which the reference
