@@ -25,7 +25,8 @@ Further legs, reported inside `config` (each can be switched off):
     central differences;
   * em_twostream: the EM two-stream workload (256x256x128, 295,279,002
     particles), psolve1 push + zigzag current, FP32 form with int64
-    fixed-point J beside the FP64 form;
+    fixed-point J beside the FP64 form, and the whole EM step with the Yee
+    update on the device (bfield, grdcrt, push, bfield, efield);
   * ramp_densest_slab: the densest 512x512x32 z-slab of the 1G exp(z) ramp
     (474,679,216 particles), the worst-loaded share of 8 GPUs.
 
@@ -182,6 +183,26 @@ def em_leg(steps32, steps64, warmup, tile_arg=0):
         ctx.push_em(2)
         j32 = ctx.current()
         el32, st32 = time_steps(ctx, "em", steps32, warmup)
+        # the whole EM step with the fields on the device, in the reference's
+        # order (emses.F90:365-434): bfield, grdcrt, psolve1 + current fold,
+        # bfield, efield; tcs = 2 cv^2 in S-units (inital.F90:78)
+        tcs = 2.0 * su.cs
+
+        def full():
+            ctx.bfield()
+            ctx.grdcrt()
+            ctx.push_em(2)
+            ctx.bfield()
+            ctx.efield(tcs, 2.0)
+
+        for _ in range(warmup):
+            full()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps32):
+            full()
+        ctx.synchronize()
+        el_full = time.perf_counter() - t0
     finally:
         ctx.close()
     k32 = kernel_ms(st32, emg.KERNEL_PUSH)
@@ -191,7 +212,9 @@ def em_leg(steps32, steps64, warmup, tile_arg=0):
                     "ms_per_step": el32 / steps32 * 1e3, "steps": steps32, "kernel_ms": k32,
                     "bytes_per_launch": bytes32,
                     "gbs_algorithmic": bytes32 / (k32 * 1e-3) / 1e9,
-                    "hbm_frac": bytes32 / (k32 * 1e-3) / 1e9 / HBM_PEAK_GBS}}
+                    "hbm_frac": bytes32 / (k32 * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                    "device_resident_step_ms": el_full / steps32 * 1e3,
+                    "device_resident_particle_steps_per_s": npart * steps32 / el_full}}
     if steps64:
         ctx = emg.Context(n, su.q, su.qm, su.rm, counts, precision=emg.PLAIN_FP64, e0=su.e0,
                           b0=su.b0)

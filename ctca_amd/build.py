@@ -28,6 +28,7 @@ SOURCES = [
     ("k_fx.hip", []),
     ("k_adj.hip", []),
     ("k_yee.hip", ["-ffp-contract=off"]),
+    ("k_poisson.hip", []),
     ("emg_api.hip", []),
 ]
 HOST_SOURCES = [("host_init.cpp", ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-I", INC])]
@@ -67,7 +68,7 @@ def build(verbose=False):
         sys.stderr.write("\n".join(failed))
         raise RuntimeError("libemgpu build failed")
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}"] + objs + ["-o", tmp])
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}"] + objs + ["-lhipfft", "-o", tmp])
     os.replace(tmp, LIB)
     return LIB
 

@@ -348,6 +348,7 @@ int emg_destroy(emg_ctx *c) {
   dfree(B.spl);
   dfree(c->eb_dev);
   dfree(c->fe_buf);
+  es_solver_free(c);
   dfree(c->node64);
   dfree(c->node32);
   dfree(c->rho64);
@@ -638,6 +639,35 @@ int emg_get_fields(emg_ctx *c, double *eb, const int32_t lo[3], const int32_t hi
         for (int q = 0; q < 6; q++) eb[dst + q] = all[src + q];
       }
   return EMG_OK;
+}
+
+int emg_solve_es(emg_ctx *c, double rhobk) {
+  CTX_CHECK(c);
+  if (!c->rho_valid) return set_error(EMG_ERR_STATE, "rho is not deposited for the current positions");
+  if (!c->eb_dev) {  // no eb uploaded: eb(6, -1:n+1)^3 with B = 0
+    int n[3] = {c->g.nx, c->g.ny, c->g.nz};
+    size_t count = 6;
+    for (int d = 0; d < 3; d++) {
+      c->eb_lo[d] = -1;
+      c->eb_ext[d] = n[d] + 3;
+      count *= (size_t)c->eb_ext[d];
+    }
+    int rc = dalloc(&c->eb_dev, count, "eb");
+    if (rc) return rc;
+    c->eb_dev_bytes = count * sizeof(double);
+    launch_zero(c->eb_dev, c->eb_dev_bytes, c->stream);
+  }
+  int rc = solve_es(c, rhobk);
+  if (rc) return rc;
+  c->fields_set = true;
+  c->nodes_stale = false;
+  return EMG_OK;
+}
+
+int emg_get_potential(emg_ctx *c, double *phi) {
+  CTX_CHECK(c);
+  if (!phi) return set_error(EMG_ERR_ARG, "null argument");
+  return read_potential(c, phi);
 }
 
 int emg_field_energy(emg_ctx *c, double cs, double *engeb) {

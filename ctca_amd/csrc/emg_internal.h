@@ -96,6 +96,8 @@ struct Timer {
   double last_ms;
 };
 
+struct EsSolver;  // device field solve (k_poisson.hip)
+
 }  // namespace emg
 
 struct emg_ctx {
@@ -122,6 +124,7 @@ struct emg_ctx {
   int eb_lo[3], eb_ext[3];
   bool nodes_stale;  // eb changed on the device since the last relocation
   double *fe_buf;    // field-energy partials and result (lazy)
+  emg::EsSolver *es; // FFT plans and arrays of the device Poisson solve (lazy)
   double *node64;  // (nx+1)(ny+1)(nz+1) x 8 doubles, nodes 0..n
   float *node32;   // nx ny nz x 8 floats, periodic nodes
   bool fields_set;
@@ -192,6 +195,11 @@ void launch_bfield(emg_ctx *c);
 void launch_efield(emg_ctx *c, double tcs, double dlt);
 int64_t field_energy_part_need();
 void launch_field_energy(emg_ctx *c, double cs, double *part_dev, double *out_dev);
+// device ES field solve (k_poisson.hip): rho of the last deposit + rhobk ->
+// phi -> E into eb_dev -> nodes
+int solve_es(emg_ctx *c, double rhobk);
+int read_potential(emg_ctx *c, double *phi);
+void es_solver_free(emg_ctx *c);
 
 // common kernels (k_common.hip)
 void launch_relocate32(emg_ctx *c);  // node64-independent: from eb_dev

@@ -38,6 +38,7 @@ EXPORTS = (
     "emg_state_save", "emg_state_load", "emg_adj_reset", "emg_adj_step", "emg_adj_get_field",
     "emg_adj_add_density", "emg_adj_velocity_sums",
     "emg_bfield", "emg_efield", "emg_grdcrt", "emg_get_fields", "emg_field_energy",
+    "emg_solve_es", "emg_get_potential",
 )
 
 
@@ -117,6 +118,8 @@ def lib():
             "emg_grdcrt": [vp],
             "emg_get_fields": [vp, vp, i32p, i32p],
             "emg_field_energy": [vp, C.c_double, vp],
+            "emg_solve_es": [vp, C.c_double],
+            "emg_get_potential": [vp, vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -276,6 +279,15 @@ class Context:
             hi = tuple(v + 1 for v in self.n)
         out = np.zeros((hi[2] - lo[2] + 1, hi[1] - lo[1] + 1, hi[0] - lo[0] + 1, 6))
         _chk(lib().emg_get_fields(self.h, _ptr(out), _i3(lo), _i3(hi)))
+        return out
+
+    def solve_es(self, rhobk=0.0):
+        _chk(lib().emg_solve_es(self.h, float(rhobk)))
+
+    def potential(self):
+        """phi of the last device field solve, numpy [k, j, i]."""
+        out = np.zeros((self.n[2], self.n[1], self.n[0]))
+        _chk(lib().emg_get_potential(self.h, _ptr(out)))
         return out
 
     def field_energy(self, cs):

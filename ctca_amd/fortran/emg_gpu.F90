@@ -288,6 +288,19 @@ module emg_gpu
       real(c_double), value :: cs
       real(c_double), dimension(2), intent(out) :: engeb
     end function emg_field_energy
+
+    ! device ES field solve: esfld1 + esfld2 + grdcrt (poissn.F90, esfild.F90)
+    integer(c_int) function emg_solve_es(ctx, rhobk) bind(C, name="emg_solve_es")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: rhobk
+    end function emg_solve_es
+
+    integer(c_int) function emg_get_potential(ctx, phi) bind(C, name="emg_get_potential")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), dimension(*), intent(out) :: phi
+    end function emg_get_potential
   end interface
 
 contains

@@ -180,6 +180,22 @@ int emg_get_fields(emg_ctx *ctx, double *eb, const int32_t lo[3], const int32_t 
 int emg_field_energy(emg_ctx *ctx, double cs, double *engeb);
 
 /*
+ * The electrostatic field solve on the device, so an ES step can stay there:
+ * esfld1 + esfld2 + grdcrt for one periodic rank.  rho = the charge of the
+ * last deposit (emg_deposit_rho, or the FP32 push) + the uniform background
+ * rhobk per node (add_background_charge, charge.F90:414-449, with chgntr's
+ * value); phi solves -lap7 phi = rho with the 7-point Laplacian's eigenvalues
+ * sum_d (2 sin(pi k_d / n_d))^2 (inifft.F90:390-404), the zero mode dropped
+ * (poissn.F90:398-410), through real-to-complex FFTs (hipFFT, not FFTW: the
+ * round-off differs); E = -grad phi on the staggered points (esfild.F90:
+ * 351-367) is written into the device eb (B kept; an eb(6, -1:n+1)^3 with
+ * B = 0 is created if none was uploaded) and relocated to the nodes.
+ * emg_get_potential reads phi(0:nx-1, 0:ny-1, 0:nz-1) of the last solve.
+ */
+int emg_solve_es(emg_ctx *ctx, double rhobk);
+int emg_get_potential(emg_ctx *ctx, double *phi);
+
+/*
  * spsort(ps) (src/etc/spsort.F90:4-39 -> lib/psort/psort-struct.c:7-9):
  * reorder each species segment by z slab int(z), the order the reference's
  * comparator (int)(a->z - b->z) produces; inside a slab (which the reference
