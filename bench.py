@@ -489,6 +489,21 @@ def main():
     rho32 = [ctx.rho(species=k) for k in range(nsp)]
     el32, st32 = time_steps(ctx, "fp32", args.steps, args.warmup)
     ek32 = ctx.energy()
+    # the whole ES step on the device: field solve from the deposited charge
+    # (spectral Poisson, E = -grad phi, grdcrt) and the fused push + charge
+    from ctca_amd import fieldsolve
+    bk = fieldsolve.background(su.q, counts, n)
+    nfull = min(args.steps, 100)
+    for _ in range(args.warmup):
+        ctx.solve_es(bk)
+        ctx.push_es(2)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(nfull):
+        ctx.solve_es(bk)
+        ctx.push_es(2)
+    ctx.synchronize()
+    el_full = time.perf_counter() - t0
     ctx.close()
     v32 = npart * args.steps / el32
     push_ms, push_n = st32[emg.KERNEL_PUSH]
@@ -579,6 +594,12 @@ def main():
                    "form": "FP32 cell index + in-cell offset push, int64 fixed-point CIC charge "
                            f"(2^24 per unit weight), tile {tile}^3 cells",
                    "kinetic_energy_fp32": ek32.tolist(),
+                   "device_resident_step": {
+                       "ms_per_step": el_full / nfull * 1e3, "steps": nfull,
+                       "particle_steps_per_s": npart * nfull / el_full,
+                       "what": "emg_solve_es (charge + background -> hipFFT Poisson -> E -> "
+                               "grdcrt) + emg_push_es (fused push + charge), fields on the "
+                               "device"},
                    "fp64": fp64,
                    "adjoint": adj,
                    "cpu_all_cores": cpu_all,
