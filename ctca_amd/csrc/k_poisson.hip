@@ -71,9 +71,11 @@ __global__ __launch_bounds__(kBlock) void k_poisson_divide(double2 *spec, Grid g
   spec[o] = v;
 }
 
+// index modulo n: one compare and add each way, the division only if needed
 __device__ __forceinline__ int pm(int i, int n) {
-  int r = i % n;
-  return r < 0 ? r + n : r;
+  int r = i < 0 ? i + n : (i >= n ? i - n : i);
+  if ((unsigned)r >= (unsigned)n) r = ((i % n) + n) % n;
+  return r;
 }
 
 __global__ __launch_bounds__(kBlock) void k_es_efield(const double *phi, double *eb, int lx,

@@ -26,9 +26,12 @@ struct EbView {
   __device__ __forceinline__ int64_t img(int c, int i, int j, int k) const {
     return at(c, pm(i, g.nx), pm(j, g.ny), pm(k, g.nz));
   }
+  // index modulo n for the indices the kernels form (a few cells around
+  // [0, n)): one compare and add each way, the division only if that misses
   __device__ __forceinline__ static int pm(int i, int n) {
-    int r = i % n;
-    return r < 0 ? r + n : r;
+    int r = i < 0 ? i + n : (i >= n ? i - n : i);
+    if ((unsigned)r >= (unsigned)n) r = ((i % n) + n) % n;
+    return r;
   }
 };
 
