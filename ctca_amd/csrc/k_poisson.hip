@@ -91,8 +91,7 @@ __global__ __launch_bounds__(kBlock) void k_es_efield(const double *phi, double 
   };
   const double p0 = P(i, j, k);
   double *e = eb + o * 6;
-  e[0] = p0 - P(i + 1, j, k);
-  e[1] = p0 - P(i, j + 1, k);
+  *reinterpret_cast<double2 *>(e) = make_double2(p0 - P(i + 1, j, k), p0 - P(i, j + 1, k));
   e[2] = p0 - P(i, j, k + 1);
 }
 

@@ -37,6 +37,20 @@ struct EbView {
 
 constexpr int BX = 3, BY = 4, BZ = 5;  // eb components: EX EY EZ BX BY BZ
 
+// A point's record is 6 doubles (48 bytes, 16-byte aligned): its E or B
+// triple is read and written as one 16-byte and one 8-byte access.
+struct D3 {
+  double x, y, z;
+};
+__device__ __forceinline__ D3 ld_e(const double *p) {
+  const double2 a = *reinterpret_cast<const double2 *>(p);
+  return {a.x, a.y, p[2]};
+}
+__device__ __forceinline__ D3 ld_b(const double *p) {
+  const double2 a = *reinterpret_cast<const double2 *>(p + 4);
+  return {p[3], a.x, a.y};
+}
+
 __global__ __launch_bounds__(kBlock) void k_bfield(EbView v) {
   const int64_t n = (int64_t)v.ext[0] * v.ext[1] * v.ext[2];
   const int64_t o = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -44,14 +58,15 @@ __global__ __launch_bounds__(kBlock) void k_bfield(EbView v) {
   const int i = (int)(o % v.ext[0]) + v.lo[0];
   const int j = (int)((o / v.ext[0]) % v.ext[1]) + v.lo[1];
   const int k = (int)(o / ((int64_t)v.ext[0] * v.ext[1])) + v.lo[2];
-  const double *e = v.eb;
-  double *b = v.eb + v.at(0, i, j, k);
-  b[BX] = b[BX] + e[v.img(1, i, j, k + 1)] - e[v.img(1, i, j, k)] - e[v.img(2, i, j + 1, k)] +
-          e[v.img(2, i, j, k)];
-  b[BY] = b[BY] + e[v.img(2, i + 1, j, k)] - e[v.img(2, i, j, k)] - e[v.img(0, i, j, k + 1)] +
-          e[v.img(0, i, j, k)];
-  b[BZ] = b[BZ] + e[v.img(0, i, j + 1, k)] - e[v.img(0, i, j, k)] - e[v.img(1, i + 1, j, k)] +
-          e[v.img(1, i, j, k)];
+  double *p = v.eb + v.at(0, i, j, k);
+  const D3 e0 = ld_e(v.eb + v.img(0, i, j, k)), ek = ld_e(v.eb + v.img(0, i, j, k + 1)),
+           ej = ld_e(v.eb + v.img(0, i, j + 1, k)), ei = ld_e(v.eb + v.img(0, i + 1, j, k));
+  const D3 b = ld_b(p);
+  const double bx = b.x + ek.y - e0.y - ej.z + e0.z;
+  const double by = b.y + ei.z - e0.z - ek.x + e0.x;
+  const double bz = b.z + ej.x - e0.x - ei.y + e0.y;
+  p[BX] = bx;
+  *reinterpret_cast<double2 *>(p + BY) = make_double2(by, bz);
 }
 
 struct CurrentView {
@@ -85,22 +100,17 @@ __global__ __launch_bounds__(kBlock) void k_efield(EbView v, CurrentView J, doub
   const int i = (int)(o % v.ext[0]) + v.lo[0];
   const int j = (int)((o / v.ext[0]) % v.ext[1]) + v.lo[1];
   const int k = (int)(o / ((int64_t)v.ext[0] * v.ext[1])) + v.lo[2];
-  const double *b = v.eb;
-  double *e = v.eb + v.at(0, i, j, k);
+  double *p = v.eb + v.at(0, i, j, k);
   const int64_t c = ((int64_t)EbView::pm(k, v.g.nz) * v.g.ny + EbView::pm(j, v.g.ny)) * v.g.nx +
                     EbView::pm(i, v.g.nx);
-  e[0] = e[0] +
-         tcs * (b[v.img(BZ, i, j, k)] - b[v.img(BZ, i, j - 1, k)] - b[v.img(BY, i, j, k)] +
-                b[v.img(BY, i, j, k - 1)]) -
-         dlt * aj_at<FIX>(J, 0, c);
-  e[1] = e[1] +
-         tcs * (b[v.img(BX, i, j, k)] - b[v.img(BX, i, j, k - 1)] - b[v.img(BZ, i, j, k)] +
-                b[v.img(BZ, i - 1, j, k)]) -
-         dlt * aj_at<FIX>(J, 1, c);
-  e[2] = e[2] +
-         tcs * (b[v.img(BY, i, j, k)] - b[v.img(BY, i - 1, j, k)] - b[v.img(BX, i, j, k)] +
-                b[v.img(BX, i, j - 1, k)]) -
-         dlt * aj_at<FIX>(J, 2, c);
+  const D3 b0 = ld_b(v.eb + v.img(0, i, j, k)), bj = ld_b(v.eb + v.img(0, i, j - 1, k)),
+           bk = ld_b(v.eb + v.img(0, i, j, k - 1)), bi = ld_b(v.eb + v.img(0, i - 1, j, k));
+  const D3 e = ld_e(p);
+  const double ex = e.x + tcs * (b0.z - bj.z - b0.y + bk.y) - dlt * aj_at<FIX>(J, 0, c);
+  const double ey = e.y + tcs * (b0.x - bk.x - b0.z + bi.z) - dlt * aj_at<FIX>(J, 1, c);
+  const double ez = e.z + tcs * (b0.y - bi.y - b0.x + bj.x) - dlt * aj_at<FIX>(J, 2, c);
+  *reinterpret_cast<double2 *>(p) = make_double2(ex, ey);
+  p[2] = ez;
 }
 
 // sum E^2 and B^2 over the cells 0..n-1: per-block partials, then one block
