@@ -28,7 +28,10 @@ Further legs, reported inside `config` (each can be switched off):
     fixed-point J beside the FP64 form, and the whole EM step with the Yee
     update on the device (bfield, grdcrt, push, bfield, efield);
   * ramp_densest_slab: the densest 512x512x32 z-slab of the 1G exp(z) ramp
-    (474,679,216 particles), the worst-loaded share of 8 GPUs.
+    (474,679,216 particles), the worst-loaded share of 8 GPUs;
+  * energy_drift: the plain ES configuration (32^3, 1,048,576 particles) run
+    self-consistently on the device (deposit, field solve, push) in both
+    forms, with the drift of kinetic + field energy.
 
 Prints one JSON line.  `roofline` is the fused FP32 kernel: algorithmic bytes
 per launch (56 B per particle + 24 B per field node + 8 B per charge node and
@@ -298,6 +301,42 @@ def slab_leg(steps32, steps64, warmup, tile_arg=0):
     return out
 
 
+def energy_drift(steps):
+    """The plain ES configuration (32^3, 2 x 524,288 particles, dt 0.1) run
+    self-consistently on the device for `steps` steps in both forms (deposit,
+    emg_solve_es, push): total energy W = kinetic + field (0.5 sum E^2 / V)
+    per step, and its largest relative change from the first step."""
+    from ctca_amd import emg, fieldsolve, units
+    w = units.es32()
+    su = units.to_sunits(w)
+    n = w.n
+    counts = su.npin
+    spec = units.device_species(w, su)
+    bk = fieldsolve.background(su.q, counts, n)
+    out = {"workload": w.name, "steps": steps, "particles": int(counts.sum())}
+    for name, prec in (("fp64", emg.PLAIN_FP64), ("fp32", emg.FP32_FIXED)):
+        ctx = emg.Context(n, su.q, su.qm, su.rm, counts, precision=prec, e0=su.e0, b0=su.b0)
+        try:
+            ctx.init_device(spec, w.seed)
+            if prec == emg.PLAIN_FP64:
+                ctx.sort()
+            ctx.deposit_rho(0)
+            ws = []
+            for _ in range(steps):
+                if prec == emg.PLAIN_FP64:
+                    ctx.deposit_rho(0)
+                ctx.solve_es(bk)
+                fe = ctx.field_energy(su.cs)
+                ctx.push_es(2)
+                ws.append(float(ctx.energy().sum() + fe[0]))
+        finally:
+            ctx.close()
+        ws = np.array(ws)
+        out[name] = {"energy_first": ws[0], "energy_last": ws[-1],
+                     "max_rel_drift": float(np.abs(ws - ws[0]).max() / abs(ws[0]))}
+    return out
+
+
 def cpu_sample(w, su, nsample):
     """A bounded sample of the workload for the oracle: nsample particles split
     over the species, each species ordered by the reference's own
@@ -454,6 +493,8 @@ def main():
     ap.add_argument("--no-em", action="store_true")
     ap.add_argument("--em-steps", type=int, default=40)
     ap.add_argument("--no-slab", action="store_true")
+    ap.add_argument("--no-drift", action="store_true")
+    ap.add_argument("--drift-steps", type=int, default=100)
     ap.add_argument("--slab-steps", type=int, default=40)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -551,6 +592,12 @@ def main():
             f"(x{adj['adjoint_over_forward']:.2f}); grad rel err {adj['rel_err_vd']:.1e}, "
             f"{adj['rel_err_E0']:.1e}")
 
+    drift = None
+    if not args.no_drift:
+        drift = energy_drift(args.drift_steps)
+        log(f"[bench] energy drift over {args.drift_steps} self-consistent steps: fp64 "
+            f"{drift['fp64']['max_rel_drift']:.2e}, fp32 {drift['fp32']['max_rel_drift']:.2e}")
+
     em = None
     if not args.no_em:
         em = em_leg(args.em_steps, max(args.em_steps // 4, 5) if not args.no_fp64 else 0,
@@ -603,6 +650,7 @@ def main():
                    "fp64": fp64,
                    "adjoint": adj,
                    "cpu_all_cores": cpu_all,
+                   "energy_drift": drift,
                    "em_twostream": em,
                    "ramp_densest_slab": slab},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
