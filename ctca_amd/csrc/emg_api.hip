@@ -623,22 +623,16 @@ int emg_get_fields(emg_ctx *c, double *eb, const int32_t lo[3], const int32_t hi
   for (int d = 0; d < 3; d++)
     if (lo[d] < c->eb_lo[d] || hi[d] > c->eb_lo[d] + c->eb_ext[d] - 1 || hi[d] < lo[d])
       return set_error(EMG_ERR_ARG, "eb bounds outside the uploaded array");
-  const size_t total = (size_t)6 * c->eb_ext[0] * c->eb_ext[1] * c->eb_ext[2];
-  std::vector<double> all(total);
-  rc = hip_check(hipStreamSynchronize(c->stream), "stream synchronize");
-  if (!rc) rc = hip_check(hipMemcpy(all.data(), c->eb_dev, sizeof(double) * total, hipMemcpyDeviceToHost),
-                          "eb download");
-  if (rc) return rc;
-  const int ex = hi[0] - lo[0] + 1, ey = hi[1] - lo[1] + 1;
-  for (int k = lo[2]; k <= hi[2]; k++)
-    for (int j = lo[1]; j <= hi[1]; j++)
-      for (int i = lo[0]; i <= hi[0]; i++) {
-        const size_t src = ((((size_t)(k - c->eb_lo[2]) * c->eb_ext[1] + (j - c->eb_lo[1])) *
-                                 c->eb_ext[0]) + (i - c->eb_lo[0])) * 6;
-        const size_t dst = ((((size_t)(k - lo[2]) * ey + (j - lo[1])) * ex) + (i - lo[0])) * 6;
-        for (int q = 0; q < 6; q++) eb[dst + q] = all[src + q];
-      }
-  return EMG_OK;
+  const int l[3] = {lo[0], lo[1], lo[2]};
+  const int ext[3] = {hi[0] - lo[0] + 1, hi[1] - lo[1] + 1, hi[2] - lo[2] + 1};
+  const size_t count = (size_t)6 * ext[0] * ext[1] * ext[2];
+  double *d_out = nullptr;
+  if ((rc = dalloc(&d_out, count, "eb read-out"))) return rc;
+  launch_eb_box(c, l, ext, d_out);
+  rc = check_flags(c);
+  if (!rc) rc = hip_check(hipMemcpy(eb, d_out, sizeof(double) * count, hipMemcpyDeviceToHost), "eb download");
+  dfree(d_out);
+  return rc;
 }
 
 int emg_solve_es(emg_ctx *c, double rhobk) {

@@ -193,6 +193,7 @@ void launch_push_em64(emg_ctx *c, int ustep);
 // device Yee update and field energy on eb_dev (k_yee.hip)
 void launch_bfield(emg_ctx *c);
 void launch_efield(emg_ctx *c, double tcs, double dlt);
+void launch_eb_box(emg_ctx *c, const int lo[3], const int ext[3], double *out_dev);
 int64_t field_energy_part_need();
 void launch_field_energy(emg_ctx *c, double cs, double *part_dev, double *out_dev);
 // device ES field solve (k_poisson.hip): rho of the last deposit + rhobk ->

@@ -156,6 +156,20 @@ __global__ void k_field_energy_final(const double *part, double cs, double vol, 
   out[1] = 0.5 * cs * tb / vol;
 }
 
+// copy the box lo .. lo + ext - 1 of eb (inside the stored bounds) into a
+// dense out(6, box)
+__global__ __launch_bounds__(kBlock) void k_eb_box(EbView v, int lx, int ly, int lz, int ex, int ey,
+                                                   int ez, double *out) {
+  const int64_t n = (int64_t)ex * ey * ez;
+  const int64_t o = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (o >= n) return;
+  const int i = (int)(o % ex) + lx, j = (int)((o / ex) % ey) + ly,
+            k = (int)(o / ((int64_t)ex * ey)) + lz;
+  const double *p = v.eb + v.at(0, i, j, k);
+#pragma unroll
+  for (int q = 0; q < 6; q++) out[o * 6 + q] = p[q];
+}
+
 EbView view_of(emg_ctx *c) {
   EbView v;
   v.eb = c->eb_dev;
@@ -193,6 +207,13 @@ void launch_efield(emg_ctx *c, double tcs, double dlt) {
   else
     hipLaunchKernelGGL(k_efield<false>, dim3(blocks_of(v)), dim3(kBlock), 0, c->stream, v, J, tcs,
                        dlt);
+}
+
+void launch_eb_box(emg_ctx *c, const int lo[3], const int ext[3], double *out_dev) {
+  const EbView v = view_of(c);
+  const int64_t n = (int64_t)ext[0] * ext[1] * ext[2];
+  hipLaunchKernelGGL(k_eb_box, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                     c->stream, v, lo[0], lo[1], lo[2], ext[0], ext[1], ext[2], out_dev);
 }
 
 int64_t field_energy_part_need() { return 2 * kFeBlocks; }
