@@ -26,7 +26,10 @@
  *   psolve1 + fold      : discrete continuity rho_new - rho_old = -ustep div J
  *                         (the reference's own rhoupd identity).
  *   energy              : cold-plasma closed form.
+ *   field_energy        : single-mode closed forms (A^2 / 4, cs B^2 / 4).
  *   chgntr              : uniform value -sum(q npin)/ncells.
+ *   fields.py (Yee)     : div B kept, Gauss's law kept, and a vacuum plane wave
+ *                         that follows the Yee dispersion sin(w/2) = 2 cv sin(k/2).
  */
 #ifndef EMSES_ORACLE_H
 #define EMSES_ORACLE_H

@@ -160,3 +160,47 @@ def test_yee_divergence_free_b_and_gauss_law():
     assert np.allclose(divb(b1), divb(ebc), atol=1e-12)
     e1 = fields.efield_full(b1, np.zeros(ebc.shape[:3] + (3,)), tcs=3.0)
     assert np.allclose(dive(e1), dive(b1), atol=1e-12)
+
+
+@pytest.mark.parametrize("m", [1, 3])
+def test_yee_vacuum_wave_follows_the_discrete_dispersion(m):
+    """A transverse plane wave E_y = cos(k x), B = 0, in vacuum: the leapfrog
+    of bfield (half step, twice) and efield (tcs = 2 cv^2, no current) must
+    oscillate with sin(w / 2) = 2 cv sin(k / 2) -- the Yee dispersion for
+    c dt / dx = 2 cv.  The mode amplitude a_n obeys a_{n+1} + a_{n-1} =
+    2 cos(w) a_n; a wrong sign in either update makes it grow instead."""
+    n = (16, 4, 4)
+    cv = 0.2
+    tcs = 2.0 * cv * cv
+    k = 2.0 * np.pi * m / n[0]
+    x = np.arange(n[0])
+    ebc = np.zeros((n[2], n[1], n[0], 6))
+    ebc[..., 1] = np.cos(k * x)[None, None, :]
+    mode = np.cos(k * x) / (n[0] / 2.0)
+    amp = [float((ebc[0, 0, :, 1] * mode).sum())]
+    zero = np.zeros(ebc.shape[:3] + (3,))
+    for _ in range(40):
+        ebc = fields.bfield_half(ebc)
+        ebc = fields.bfield_half(ebc)
+        ebc = fields.efield_full(ebc, zero, tcs)
+        amp.append(float((ebc[0, 0, :, 1] * mode).sum()))
+    a = np.array(amp)
+    cosw = 1.0 - 8.0 * cv * cv * np.sin(k / 2.0) ** 2
+    assert np.allclose(a[2:] + a[:-2], 2.0 * cosw * a[1:-1], atol=1e-12)
+    # a stable oscillation (the B = 0 start adds a small sine part): bounded
+    assert np.abs(a).max() <= 1.05 and a.min() < -0.5
+
+
+def test_field_energy_single_mode_closed_form():
+    """energy.F90:52-70: 0.5 sum E^2 / V and 0.5 cs sum B^2 / V of single modes
+    A cos(k x): A^2 / 4 and cs B^2 / 4."""
+    n = (16, 8, 4)
+    x = np.arange(n[0])
+    ebc = np.zeros((n[2], n[1], n[0], 6))
+    ebc[..., 0] = 0.3 * np.cos(2 * np.pi * 2 * x / n[0])[None, None, :]
+    ebc[..., 5] = 0.7 * np.sin(2 * np.pi * 3 * x / n[0])[None, None, :]
+    eb = np.pad(ebc, [(1, 2), (1, 2), (1, 2), (0, 0)], mode="wrap")
+    cs = 0.04
+    out = oracle.field_energy(eb, n, cs, np.array(n, dtype=float))
+    assert out[0] == pytest.approx(0.3 ** 2 / 4.0, rel=1e-13)
+    assert out[1] == pytest.approx(cs * 0.7 ** 2 / 4.0, rel=1e-13)
