@@ -348,6 +348,7 @@ int emg_destroy(emg_ctx *c) {
   dfree(B.spl);
   dfree(c->eb_dev);
   dfree(c->fe_buf);
+  dfree(c->sort_rec);
   es_solver_free(c);
   dfree(c->node64);
   dfree(c->node32);

@@ -148,6 +148,7 @@ struct emg_ctx {
   int32_t *sort_cursor;
   int64_t *sort_start;
   int64_t sort_len;
+  void *sort_rec;        // cap_total x 64-byte records + cap_total int32 ranks (FP64 sort, lazy)
   int64_t *scan_tmp;
   int64_t scan_tmp_len;
 

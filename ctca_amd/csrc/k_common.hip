@@ -228,31 +228,58 @@ __device__ __forceinline__ int64_t slab_key(const State64 &s, int64_t m, const G
   return ((int64_t)k * g.ny + j) * g.nx + i;
 }
 
+// count the particles of each key; the count a particle finds is its rank
+// among the key's particles (kept for the scatter)
 __global__ __launch_bounds__(kBlock) void k_sort_hist(State64 s, SpeciesTab sp, Grid g, int64_t nkey,
-                                                      int32_t *count) {
+                                                      int32_t *count, int32_t *rank) {
   int is = blockIdx.y;
   int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (m >= sp.cnt[is]) return;
   m += sp.off[is];
-  atomicAdd(&count[(int64_t)is * nkey + slab_key(s, m, g)], 1);
+  rank[m] = atomicAdd(&count[(int64_t)is * nkey + slab_key(s, m, g)], 1);
 }
 
-__global__ __launch_bounds__(kBlock) void k_sort_scatter(State64 s, State64 t, SpeciesTab sp, Grid g,
-                                                         int64_t nkey, const int64_t *start,
-                                                         int32_t *cursor) {
+// The destinations of a sort are a random permutation: written straight into
+// the seven state arrays, every particle dirties seven cache lines at random.
+// Instead each particle goes to its slot as one 64-byte record (one line),
+// and a second, coalesced pass splits the records into the arrays.
+__global__ __launch_bounds__(kBlock) void k_sort_scatter(State64 s, uint4 *rec, SpeciesTab sp,
+                                                         Grid g, int64_t nkey,
+                                                         const int64_t *start, const int32_t *rank) {
   int is = blockIdx.y;
   int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (m >= sp.cnt[is]) return;
   m += sp.off[is];
   int64_t key = (int64_t)is * nkey + slab_key(s, m, g);
-  int64_t d = sp.off[is] + (start[key] - start[(int64_t)is * nkey]) + atomicAdd(&cursor[key], 1);
-  t.x[d] = s.x[m];
-  t.y[d] = s.y[m];
-  t.z[d] = s.z[m];
-  t.vx[d] = s.vx[m];
-  t.vy[d] = s.vy[m];
-  t.vz[d] = s.vz[m];
-  t.pid[d] = s.pid[m];
+  int64_t d = sp.off[is] + (start[key] - start[(int64_t)is * nkey]) + rank[m];
+  auto u2 = [](double a, double b) {
+    const unsigned long long ia = (unsigned long long)__double_as_longlong(a),
+                             ib = (unsigned long long)__double_as_longlong(b);
+    return make_uint4((unsigned)ia, (unsigned)(ia >> 32), (unsigned)ib, (unsigned)(ib >> 32));
+  };
+  uint4 *r = rec + 4 * d;
+  r[0] = u2(s.x[m], s.y[m]);
+  r[1] = u2(s.z[m], s.vx[m]);
+  r[2] = u2(s.vy[m], s.vz[m]);
+  r[3] = make_uint4((unsigned)s.pid[m], 0u, 0u, 0u);
+}
+
+__global__ __launch_bounds__(kBlock) void k_sort_split(const uint4 *rec, State64 t, SpeciesTab sp) {
+  int is = blockIdx.y;
+  int64_t d = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (d >= sp.cnt[is]) return;
+  d += sp.off[is];
+  const uint4 *r = rec + 4 * d;
+  auto lo = [](uint4 v) { return __longlong_as_double((long long)(((unsigned long long)v.y << 32) | v.x)); };
+  auto hi = [](uint4 v) { return __longlong_as_double((long long)(((unsigned long long)v.w << 32) | v.z)); };
+  const uint4 a = r[0], b = r[1], c = r[2], e = r[3];
+  t.x[d] = lo(a);
+  t.y[d] = hi(a);
+  t.z[d] = lo(b);
+  t.vx[d] = hi(b);
+  t.vy[d] = lo(c);
+  t.vz[d] = hi(c);
+  t.pid[d] = (int32_t)e.x;
 }
 
 // ---------------------------------------------------------------- device init
@@ -427,15 +454,21 @@ int sort64(emg_ctx *c) {
     c->t64 = State64{(double *)p[0], (double *)p[1], (double *)p[2], (double *)p[3],
                      (double *)p[4], (double *)p[5], (int32_t *)p[6]};
   }
+  if (!c->sort_rec && hipMalloc(&c->sort_rec, 68 * (size_t)c->cap_total) != hipSuccess) {
+    c->sort_rec = nullptr;
+    return set_error(EMG_ERR_NOMEM, "sort records");
+  }
+  int32_t *rank = reinterpret_cast<int32_t *>(static_cast<char *>(c->sort_rec) + 64 * (size_t)c->cap_total);
   launch_zero(c->sort_count, sizeof(int32_t) * len, c->stream);
-  launch_zero(c->sort_cursor, sizeof(int32_t) * len, c->stream);
   dim3 grid = species_grid(c->sp);
   timer_begin(c, kTSort);
   hipLaunchKernelGGL(k_sort_hist, grid, dim3(kBlock), 0, c->stream, c->s64, c->sp, c->g, nkey,
-                     c->sort_count);
+                     c->sort_count, rank);
   exclusive_scan_i32(c->sort_count, c->sort_start, len, c->scan_tmp, c->stream);
-  hipLaunchKernelGGL(k_sort_scatter, grid, dim3(kBlock), 0, c->stream, c->s64, c->t64, c->sp, c->g,
-                     nkey, c->sort_start, c->sort_cursor);
+  hipLaunchKernelGGL(k_sort_scatter, grid, dim3(kBlock), 0, c->stream, c->s64,
+                     (uint4 *)c->sort_rec, c->sp, c->g, nkey, c->sort_start, rank);
+  hipLaunchKernelGGL(k_sort_split, grid, dim3(kBlock), 0, c->stream, (const uint4 *)c->sort_rec,
+                     c->t64, c->sp);
   timer_end(c, kTSort);
   State64 tmp = c->s64;
   c->s64 = c->t64;
