@@ -350,6 +350,7 @@ int emg_destroy(emg_ctx *c) {
   dfree(c->fe_buf);
   dfree(c->sort_rec);
   es_solver_free(c);
+  dfree(c->bk_dev);
   dfree(c->node64);
   dfree(c->node32);
   dfree(c->rho64);
@@ -657,6 +658,22 @@ int emg_solve_es(emg_ctx *c, double rhobk) {
   c->fields_set = true;
   c->nodes_stale = false;
   return EMG_OK;
+}
+
+int emg_set_background(emg_ctx *c, const double *rhobk) {
+  CTX_CHECK(c);
+  if (!rhobk) {
+    dfree(c->bk_dev);
+    c->bk_dev = nullptr;
+    return EMG_OK;
+  }
+  int rc;
+  if (!c->bk_dev && (rc = dalloc(&c->bk_dev, (size_t)c->ncell, "background charge"))) return rc;
+  rc = hip_check(hipMemcpyAsync(c->bk_dev, rhobk, sizeof(double) * c->ncell,
+                                hipMemcpyHostToDevice, c->stream),
+                 "background copy");
+  if (!rc) rc = hip_check(hipStreamSynchronize(c->stream), "background copy");
+  return rc;
 }
 
 int emg_get_potential(emg_ctx *c, double *phi) {

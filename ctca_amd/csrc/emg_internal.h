@@ -125,6 +125,7 @@ struct emg_ctx {
   bool nodes_stale;  // eb changed on the device since the last relocation
   double *fe_buf;    // field-energy partials and result (lazy)
   emg::EsSolver *es; // FFT plans and arrays of the device Poisson solve (lazy)
+  double *bk_dev;    // background charge per periodic node (emg_set_background), or null
   double *node64;  // (nx+1)(ny+1)(nz+1) x 8 doubles, nodes 0..n
   float *node32;   // nx ny nz x 8 floats, periodic nodes
   bool fields_set;

@@ -1,7 +1,8 @@
 // k_poisson.hip -- the electrostatic field solve on the device, for one
 // periodic rank, so an ES step can stay on the device:
-//   rho    = the deposited charge of every species + a uniform background
-//            (charge.F90:414-449 add_background_charge with chgntr's value);
+//   rho    = the deposited charge of every species + the background: the
+//            per-node array of emg_set_background, if any, then the uniform
+//            value (charge.F90:414-449 add_background_charge);
 //   phi    : -lap7 phi = rho, spectrally: the 7-point Laplacian has the
 //            eigenvalues kappa = sum_d (2 sin(pi k_d / n_d))^2
 //            (inifft.F90:390-404), the zero mode is dropped
@@ -33,7 +34,8 @@ namespace {
 
 __global__ __launch_bounds__(kBlock) void k_rho_total(const double *rho64,
                                                       const unsigned long long *rhofx,
-                                                      SpeciesTab sp, int64_t ncell, double bk,
+                                                      SpeciesTab sp, int64_t ncell,
+                                                      const double *bkarr, double bk,
                                                       double *out) {
   const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (c >= ncell) return;
@@ -44,6 +46,7 @@ __global__ __launch_bounds__(kBlock) void k_rho_total(const double *rho64,
     else
       v = v + sp.q[s] * ((double)(long long)rhofx[(int64_t)s * ncell + c] * (1.0 / kFixScale));
   }
+  if (bkarr) v = v + bkarr[c];
   out[c] = v + bk;
 }
 
@@ -147,7 +150,7 @@ int solve_es(emg_ctx *c, double rhobk) {
   const unsigned gc = (unsigned)((c->ncell + kBlock - 1) / kBlock);
   const bool fix = c->cfg.precision == EMG_FP32_FIXED;
   hipLaunchKernelGGL(k_rho_total, dim3(gc), dim3(kBlock), 0, c->stream, fix ? nullptr : c->rho64,
-                     fix ? c->rhofx : nullptr, c->sp, c->ncell, rhobk, s->rho);
+                     fix ? c->rhofx : nullptr, c->sp, c->ncell, c->bk_dev, rhobk, s->rho);
   rc = fft_check(hipfftExecD2Z(s->fwd, s->rho, reinterpret_cast<hipfftDoubleComplex *>(s->spec)),
                  "D2Z");
   if (rc) return rc;

@@ -38,7 +38,7 @@ EXPORTS = (
     "emg_state_save", "emg_state_load", "emg_adj_reset", "emg_adj_step", "emg_adj_get_field",
     "emg_adj_add_density", "emg_adj_velocity_sums",
     "emg_bfield", "emg_efield", "emg_grdcrt", "emg_get_fields", "emg_field_energy",
-    "emg_solve_es", "emg_get_potential",
+    "emg_solve_es", "emg_get_potential", "emg_set_background",
 )
 
 
@@ -120,6 +120,7 @@ def lib():
             "emg_field_energy": [vp, C.c_double, vp],
             "emg_solve_es": [vp, C.c_double],
             "emg_get_potential": [vp, vp],
+            "emg_set_background": [vp, vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -283,6 +284,18 @@ class Context:
 
     def solve_es(self, rhobk=0.0):
         _chk(lib().emg_solve_es(self.h, float(rhobk)))
+
+    def set_background(self, rhobk):
+        """Per-node background charge [k, j, i] over the periodic nodes, added
+        by every following solve_es (None removes it)."""
+        if rhobk is None:
+            _chk(lib().emg_set_background(self.h, None))
+            return
+        a = np.ascontiguousarray(rhobk, dtype=np.float64)
+        if a.shape != (self.n[2], self.n[1], self.n[0]):
+            raise ValueError(f"background shape {a.shape} is not (nz, ny, nx) = "
+                             f"{(self.n[2], self.n[1], self.n[0])}")
+        _chk(lib().emg_set_background(self.h, _ptr(a)))
 
     def potential(self):
         """phi of the last device field solve, numpy [k, j, i]."""

@@ -301,6 +301,13 @@ module emg_gpu
       type(c_ptr), value :: ctx
       real(c_double), dimension(*), intent(out) :: phi
     end function emg_get_potential
+
+    ! background charge per periodic node, rhobk(1,0:nx-1,0:ny-1,0:nz-1,3)
+    integer(c_int) function emg_set_background(ctx, rhobk) bind(C, name="emg_set_background")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), dimension(*), intent(in) :: rhobk
+    end function emg_set_background
   end interface
 
 contains

@@ -193,6 +193,15 @@ int emg_field_energy(emg_ctx *ctx, double cs, double *engeb);
  * emg_get_potential reads phi(0:nx-1, 0:ny-1, 0:nz-1) of the last solve.
  */
 int emg_solve_es(emg_ctx *ctx, double rhobk);
+
+/*
+ * A background charge that varies in space: rhobk(1, 0:nx-1, 0:ny-1, 0:nz-1,
+ * 3) of the reference (x fastest; node n_d is the periodic image of node 0),
+ * added per node by every following emg_solve_es before its uniform rhobk
+ * (add_background_charge, charge.F90:440-447; the array is what a snapshot
+ * stores as 'rhobk', snap.F90:334-337).  A null pointer removes it.
+ */
+int emg_set_background(emg_ctx *ctx, const double *rhobk);
 int emg_get_potential(emg_ctx *ctx, double *phi);
 
 /*
