@@ -233,7 +233,9 @@ int emg_create(const emg_config *cfg, emg_ctx **out) {
     if (cfg->n[d] < 2) return set_error(EMG_ERR_ARG, "need at least 2 cells per axis");
   if (cfg->precision != EMG_PLAIN_FP64 && cfg->precision != EMG_FP32_FIXED)
     return set_error(EMG_ERR_ARG, "unknown precision form");
-  if (cfg->reserved != 0) return set_error(EMG_ERR_ARG, "reserved field must be 0");
+  if (cfg->sort_every < 0) return set_error(EMG_ERR_ARG, "sort_every must be >= 0");
+  if (cfg->sort_every != 0 && cfg->precision != EMG_PLAIN_FP64)
+    return set_error(EMG_ERR_ARG, "sort_every applies to EMG_PLAIN_FP64 only");
   for (int s = 0; s < cfg->nspec; s++)
     if (cfg->capacity[s] < 0) return set_error(EMG_ERR_ARG, "negative capacity");
 
@@ -488,6 +490,7 @@ int emg_upload_particles(emg_ctx *c, const emg_particle *pbuf, const int64_t *co
     return rc;
   }
   c->have_particles = true;
+  c->pushes_since_sort = c->cfg.sort_every;  // unsorted: sort before the first push
   c->rho_valid = false;
   c->j_valid = false;
   c->ke_valid = false;
@@ -709,11 +712,24 @@ static int need_fields(emg_ctx *c) {
   return EMG_OK;
 }
 
+// PLAIN_FP64 with sort_every > 0: the cell sort that is due before a push
+static int sort_if_due(emg_ctx *c) {
+  if (c->cfg.precision != EMG_PLAIN_FP64 || c->cfg.sort_every <= 0) return EMG_OK;
+  if (c->pushes_since_sort < c->cfg.sort_every) {
+    c->pushes_since_sort++;
+    return EMG_OK;
+  }
+  int rc = sort64(c);
+  if (!rc) c->pushes_since_sort = 1;
+  return rc;
+}
+
 int emg_positn(emg_ctx *c, int32_t ustep) {
   CTX_CHECK(c);
   int rc = need_particles(c);
   if (rc) return rc;
   if (ustep < 0) return set_error(EMG_ERR_ARG, "ustep must be >= 0");
+  if ((rc = sort_if_due(c))) return rc;
   if (c->cfg.precision == EMG_PLAIN_FP64)
     launch_positn64(c, ustep);
   else
@@ -729,6 +745,7 @@ int emg_push_es(emg_ctx *c, int32_t ustep) {
   if (!rc) rc = need_fields(c);
   if (rc) return rc;
   if (ustep < 0) return set_error(EMG_ERR_ARG, "ustep must be >= 0");
+  if ((rc = sort_if_due(c))) return rc;
   if (c->cfg.precision == EMG_PLAIN_FP64) {
     launch_push_es64(c, ustep);
     c->rho_valid = false;
@@ -822,6 +839,7 @@ int emg_push_em(emg_ctx *c, int32_t ustep) {
   if (!rc) rc = need_fields(c);
   if (rc) return rc;
   if (ustep < 1) return set_error(EMG_ERR_ARG, "ustep must be >= 1");
+  if ((rc = sort_if_due(c))) return rc;
   if (c->cfg.precision == EMG_PLAIN_FP64) {
     launch_push_em64(c, ustep);
     c->ke_valid = false;
@@ -963,10 +981,12 @@ int emg_sort(emg_ctx *c) {
   CTX_CHECK(c);
   int rc = need_particles(c);
   if (rc) return rc;
-  if (c->cfg.precision == EMG_PLAIN_FP64)
+  if (c->cfg.precision == EMG_PLAIN_FP64) {
     rc = sort64(c);
-  else
+    if (!rc) c->pushes_since_sort = 0;
+  } else {
     rc = fx_rebucket(c);
+  }
   return rc;
 }
 
@@ -995,6 +1015,7 @@ int emg_init_device(emg_ctx *c, const emg_device_species *spec, int32_t nspec, u
     return rc;
   }
   c->have_particles = true;
+  c->pushes_since_sort = c->cfg.sort_every;  // unsorted: sort before the first push
   c->rho_valid = false;
   c->j_valid = false;
   c->ke_valid = false;

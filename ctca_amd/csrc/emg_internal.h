@@ -114,6 +114,7 @@ struct emg_ctx {
   emg::State64 s64;
   emg::State64 t64;  // sort scratch (lazy)
   bool have_particles;
+  int pushes_since_sort;  // PLAIN_FP64 pushes since the last cell sort
 
   // FP32 state
   emg::FxBuckets fx;

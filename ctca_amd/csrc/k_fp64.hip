@@ -211,22 +211,51 @@ __global__ __launch_bounds__(kBlock) void k_push_es64(State64 s, SpeciesTab sp,
 // charge.F90:59-83 + distribute_charge 197-210; rho[is][node mod n] += v_k q(is)
 // charge.F90:152-210 per particle; the particles of a block are a contiguous
 // run of a species (kDepPer per thread).  The sort orders particles by cell
-// and they drift slowly, so most of a block's particles lie in a box around
-// its middle particle's cell: the block sums their weights in an LDS window
-// of that box, as int64 fixed point at 2^48 per unit weight (at most
-// kBlock kDepPer terms of at most 2^48 per node), and adds each non-zero
-// window node times q(is) to rho with one global FP64 atomic; a particle
-// whose eight nodes are not all in the window adds its weights to rho
-// directly.  The rounding of a weight (2^-49) is below the FP64 round-off
-// of the sequential sum, which the reference's order would give.
+// and they drift slowly, so most of a block's particles lie in a box of
+// nodes: the block sums their weights in an LDS window of that box, as int64
+// fixed point at 2^48 per unit weight (at most kBlock kDepPer terms of at
+// most 2^48 per node), and adds each non-zero window node times q(is) to rho
+// with one global FP64 atomic; a particle whose eight nodes are not all in
+// the window adds its weights to rho directly.  The rounding of a weight
+// (2^-49) is below the FP64 round-off of the sequential sum, which the
+// reference's order would give.
+//
+// The box: the cells of a sorted run go from its first to its last
+// particle's cell in z-major order, so its nodes fit a box one row segment
+// long (one row) or whole rows wide (several rows of one layer), two layers
+// deep; the rest of the window's capacity becomes margins in z, y and x for
+// the particles that have drifted since the sort.  A run that spans layers,
+// or whose end points are out of order after a long drift, gets a fixed box
+// around its middle particle.
 constexpr int kDepPer = 16;
-constexpr int kWinX = 80, kWinY = 8, kWinZ = 6;  // 30 KB of int64
+constexpr int kWinCap = 3840;                    // 30 KB of int64
+constexpr int kWinX = 80, kWinY = 8, kWinZ = 6;  // the fixed box
 constexpr double kRhoFix = 281474976710656.0;   // 2^48 per unit weight
+
+// window index of node l + 1 on an axis of window extent w (full axis: w = n,
+// where node n is node 0); false if it is outside the window
+__device__ __forceinline__ bool win_next(int l, int w, int n, int &l1) {
+  l1 = l + 1;
+  if (l1 < w) return true;
+  l1 = 0;
+  return w == n && l + 1 == n;
+}
+
+// widen one axis of the window by a node on each side (or to the whole axis)
+__device__ __forceinline__ void win_grow(int &o, int &w, int n) {
+  if (w + 2 >= n) {
+    o = 0;
+    w = n;
+  } else {
+    o = o == 0 ? n - 1 : o - 1;
+    w += 2;
+  }
+}
 
 __global__ __launch_bounds__(kBlock) void k_deposit64(State64 s, SpeciesTab sp, Grid g,
                                                       double *__restrict__ rho, int64_t ncell,
                                                       int *flag) {
-  __shared__ unsigned long long win[kWinX * kWinY * kWinZ];
+  __shared__ unsigned long long win[kWinCap];
   const int is = blockIdx.y;
   const int64_t beg = (int64_t)blockIdx.x * kBlock * kDepPer;
   const int64_t cnt = sp.cnt[is];
@@ -234,11 +263,58 @@ __global__ __launch_bounds__(kBlock) void k_deposit64(State64 s, SpeciesTab sp, 
   const int64_t end = min(beg + (int64_t)kBlock * kDepPer, cnt);
   const int64_t off = sp.off[is];
   // window: nodes o .. o + w - 1 per axis (w <= n, so no node appears twice)
-  const int wx = min(kWinX, g.nx), wy = min(kWinY, g.ny), wz = min(kWinZ, g.nz);
-  const int64_t mid = off + min(beg + (int64_t)kBlock * kDepPer / 2, end - 1);
-  const int ox = wrapn(min((int)s.x[mid], g.nx - 1) - wx / 2 + g.nx, g.nx);
-  const int oy = wrapn(min((int)s.y[mid], g.ny - 1) - wy / 2 + g.ny, g.ny);
-  const int oz = wrapn(min((int)s.z[mid], g.nz - 1) - wz / 2 + g.nz, g.nz);
+  int wx, wy, wz, ox, oy, oz;
+  {
+    auto cell = [&](double v, int n) { return min(max((int)v, 0), n - 1); };
+    const int64_t a = off + beg, b = off + end - 1;
+    const int ia = cell(s.x[a], g.nx), ja = cell(s.y[a], g.ny), ka = cell(s.z[a], g.nz);
+    const int ib = cell(s.x[b], g.nx), jb = cell(s.y[b], g.ny), kb = cell(s.z[b], g.nz);
+    bool boxed = ka == kb && (jb > ja || (jb == ja && ib >= ia));
+    if (boxed) {
+      const bool row = jb == ja;
+      ox = row ? ia : 0;
+      wx = row ? min(ib - ia + 2, g.nx) : g.nx;
+      oy = ja;
+      wy = min(jb - ja + 2, g.ny);
+      oz = ka;
+      wz = min(2, g.nz);
+      if (wx == g.nx) ox = 0;
+      if (wy == g.ny) oy = 0;
+      if (wz == g.nz) oz = 0;
+      boxed = wx * wy * wz <= kWinCap;
+    }
+    if (boxed) {  // margins while they fit: z, y, x in turn
+      int o[3] = {ox, oy, oz}, w[3] = {wx, wy, wz};
+      const int n[3] = {g.nx, g.ny, g.nz};
+      for (int d = 2, stuck = 0; stuck < 3; d = d == 0 ? 2 : d - 1) {
+        int o2 = o[d], w2 = w[d];
+        bool grew = false;
+        if (w2 < n[d]) {
+          win_grow(o2, w2, n[d]);
+          if ((int64_t)w2 * w[(d + 1) % 3] * w[(d + 2) % 3] <= kWinCap) {
+            o[d] = o2;
+            w[d] = w2;
+            grew = true;
+          }
+        }
+        stuck = grew ? 0 : stuck + 1;
+      }
+      ox = o[0];
+      oy = o[1];
+      oz = o[2];
+      wx = w[0];
+      wy = w[1];
+      wz = w[2];
+    } else {  // the fixed box around the middle particle
+      wx = min(kWinX, g.nx);
+      wy = min(kWinY, g.ny);
+      wz = min(kWinZ, g.nz);
+      const int64_t mid = off + min(beg + (int64_t)kBlock * kDepPer / 2, end - 1);
+      ox = wrapn(cell(s.x[mid], g.nx) - wx / 2 + g.nx, g.nx);
+      oy = wrapn(cell(s.y[mid], g.ny) - wy / 2 + g.ny, g.ny);
+      oz = wrapn(cell(s.z[mid], g.nz) - wz / 2 + g.nz, g.nz);
+    }
+  }
   for (int q = threadIdx.x; q < wx * wy * wz; q += kBlock) win[q] = 0ull;
   __syncthreads();
   const double qc = sp.q[is];
@@ -251,17 +327,19 @@ __global__ __launch_bounds__(kBlock) void k_deposit64(State64 s, SpeciesTab sp, 
     // window coordinates of the cell (cell n, of a position at L, is cell 0's image)
     int li = wrapn(i - ox + g.nx, g.nx), lj = wrapn(j - oy + g.ny, g.ny),
         lk = wrapn(k - oz + g.nz, g.nz);
-    if (li < wx - 1 && lj < wy - 1 && lk < wz - 1) {
-      unsigned long long *p = win + (lk * wy + lj) * wx + li;
+    int li1, lj1, lk1;
+    if (li < wx && lj < wy && lk < wz && win_next(li, wx, g.nx, li1) &&
+        win_next(lj, wy, g.ny, lj1) && win_next(lk, wz, g.nz, lk1)) {
+      auto at = [&](int a, int b, int c) { return win + (c * wy + b) * wx + a; };
       auto fx = [](double v) { return (unsigned long long)__double2ll_rn(v * kRhoFix); };
-      atomicAdd(p + wx * wy, fx(w.v1));
-      atomicAdd(p + wx * wy + 1, fx(w.v2));
-      atomicAdd(p + wx * wy + wx + 1, fx(w.v3));
-      atomicAdd(p + wx * wy + wx, fx(w.v4));
-      atomicAdd(p, fx(w.v5));
-      atomicAdd(p + 1, fx(w.v6));
-      atomicAdd(p + wx + 1, fx(w.v7));
-      atomicAdd(p + wx, fx(w.v8));
+      atomicAdd(at(li, lj, lk1), fx(w.v1));
+      atomicAdd(at(li1, lj, lk1), fx(w.v2));
+      atomicAdd(at(li1, lj1, lk1), fx(w.v3));
+      atomicAdd(at(li, lj1, lk1), fx(w.v4));
+      atomicAdd(at(li, lj, lk), fx(w.v5));
+      atomicAdd(at(li1, lj, lk), fx(w.v6));
+      atomicAdd(at(li1, lj1, lk), fx(w.v7));
+      atomicAdd(at(li, lj1, lk), fx(w.v8));
     } else {
       int i1 = i + 1, j1 = j + 1, k1 = k + 1;
       atomicAdd(r + cidx(g, i, j, k1), w.v1 * qc);

@@ -50,7 +50,7 @@ class EmgError(RuntimeError):
 
 class Config(C.Structure):
     _fields_ = [("n", C.c_int32 * 3), ("nspec", C.c_int32), ("precision", C.c_int32),
-                ("tile", C.c_int32), ("track_pid", C.c_int32), ("reserved", C.c_int32),
+                ("tile", C.c_int32), ("track_pid", C.c_int32), ("sort_every", C.c_int32),
                 ("capacity", C.c_int64 * MAX_SPECIES),
                 ("q", C.c_double * MAX_SPECIES), ("qm", C.c_double * MAX_SPECIES),
                 ("rm", C.c_double * MAX_SPECIES), ("e0", C.c_double * 3), ("b0", C.c_double * 3)]
@@ -171,7 +171,7 @@ class Context:
     """One GPU-resident particle population (include/emgpu.h emg_ctx)."""
 
     def __init__(self, n, q, qm, rm, capacity, precision=PLAIN_FP64, e0=(0, 0, 0),
-                 b0=(0, 0, 0), tile=0, track_pid=False):
+                 b0=(0, 0, 0), tile=0, track_pid=False, sort_every=0):
         ns = len(q)
         cfg = Config()
         cfg.n = (C.c_int32 * 3)(*[int(v) for v in n])
@@ -179,6 +179,7 @@ class Context:
         cfg.precision = int(precision)
         cfg.tile = int(tile)
         cfg.track_pid = 1 if track_pid else 0
+        cfg.sort_every = int(sort_every)
         for k in range(ns):
             cfg.capacity[k] = int(capacity[k])
             cfg.q[k] = float(q[k])

@@ -31,7 +31,7 @@ module emg_gpu
     integer(c_int32_t) :: precision = EMG_PLAIN_FP64
     integer(c_int32_t) :: tile = 0
     integer(c_int32_t) :: track_pid = 0
-    integer(c_int32_t) :: reserved = 0
+    integer(c_int32_t) :: sort_every = 0
     integer(c_int64_t) :: capacity(EMG_MAX_SPECIES) = 0
     real(c_double) :: q(EMG_MAX_SPECIES) = 0d0
     real(c_double) :: qm(EMG_MAX_SPECIES) = 0d0

@@ -68,7 +68,12 @@ typedef struct {
                           every axis is a multiple of 8, else 4)             */
   int32_t track_pid;   /* FP32_FIXED: carry pid through every move (+8 B per
                           particle-step); without it download returns pid 0  */
-  int32_t reserved;    /* must be 0                                          */
+  int32_t sort_every;  /* PLAIN_FP64: re-sort by cell (emg_sort) before every
+                          sort_every-th push, and before the first push
+                          after an upload; 0 = only when emg_sort is called
+                          (the reference's isort cadence).  A sort only
+                          permutes the particles.  FP32_FIXED: must be 0
+                          (its tile buckets are re-formed every step)       */
   int64_t capacity[EMG_MAX_SPECIES]; /* max particles per species            */
   double q[EMG_MAX_SPECIES];  /* charge per macro particle, allcom q(is)     */
   double qm[EMG_MAX_SPECIES]; /* charge-to-mass ratio, allcom qm(is)         */

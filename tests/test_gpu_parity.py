@@ -430,6 +430,53 @@ def test_sort_fp64_is_a_z_slab_ordered_permutation():
     assert np.abs(r1 - r0).max() <= 1e-13 * np.abs(r0).max()
 
 
+def test_sort_every_only_permutes():
+    """sort_every = 3: the FP64 form sorts itself before the first push and
+    before every third push after that; positions and velocities stay bit-identical to a
+    run that never sorts (matched by pid), the densities and currents agree
+    to round-off, and every species segment ends in the sort's order."""
+    s = SMALL
+    n = s["n"]
+    rng = np.random.default_rng(3)
+    eb = periodic_eb(n, rng, 0.05, 0.02)
+    p = small_state()
+    out = {}
+    for every in (0, 3):
+        with ctx_for(n, s["q"], s["qm"], s["rm"], s["counts"], b0=(0.0, 0.0, 0.03),
+                     sort_every=every) as ctx:
+            ctx.upload(p, s["counts"])
+            ctx.set_fields(eb)
+            for _ in range(6):
+                ctx.push_es(2)  # every = 3: sorted before the 1st and 4th
+            ctx.deposit_rho(0)
+            rho = ctx.rho()
+            ctx.push_em(2)      # the 7th push: sorted before it
+            j = ctx.current()
+            g, cnt = ctx.download()
+        out[every] = (by_pid(g), rho, j, g)
+    for f in FIELDS:
+        assert np.array_equal(out[0][0][f], out[3][0][f]), f
+    assert np.abs(out[3][1] - out[0][1]).max() <= 1e-13 * np.abs(out[0][1]).max()
+    assert np.abs(out[3][2] - out[0][2]).max() <= 1e-12 * np.abs(out[0][2]).max()
+    g = out[3][3]
+    off = 0
+    L = float(n[2])
+    for c in s["counts"]:
+        # sorted (floor z non-decreasing) before the 7th push, which moves a
+        # particle by vz ustep: neighbours in the order differ by more than
+        # -1 - 2 max|vz| ustep (a random order reaches -L/2)
+        seg = g[off:off + c]
+        d = np.diff(seg["z"])
+        d = (d + L / 2) % L - L / 2     # across the periodic face
+        assert np.all(d > -1.0 - 4.0 * np.abs(seg["vz"]).max() - 1e-12)
+        off += c
+    with pytest.raises(emg.EmgError):
+        ctx_for(n, s["q"], s["qm"], s["rm"], s["counts"], precision=emg.FP32_FIXED,
+                sort_every=3)
+    with pytest.raises(emg.EmgError):
+        ctx_for(n, s["q"], s["qm"], s["rm"], s["counts"], sort_every=-1)
+
+
 def test_push_em_fp64_vs_oracle():
     n = (12, 10, 8)
     counts = (20000, 5000)
