@@ -218,15 +218,7 @@ __global__ __launch_bounds__(kBlock) void k_push_es64(State64 s, SpeciesTab sp,
 // with one global FP64 atomic; a particle whose eight nodes are not all in
 // the window adds its weights to rho directly.  The rounding of a weight
 // (2^-49) is below the FP64 round-off of the sequential sum, which the
-// reference's order would give.
-//
-// The box: the cells of a sorted run go from its first to its last
-// particle's cell in z-major order, so its nodes fit a box one row segment
-// long (one row) or whole rows wide (several rows of one layer), two layers
-// deep; the rest of the window's capacity becomes margins in z, y and x for
-// the particles that have drifted since the sort.  A run that spans layers,
-// or whose end points are out of order after a long drift, gets a fixed box
-// around its middle particle.
+// reference's order would give.  The window's shape: run_window.
 constexpr int kDepPer = 16;
 constexpr int kWinCap = 3840;                    // 30 KB of int64
 constexpr int kWinX = 80, kWinY = 8, kWinZ = 6;  // the fixed box
@@ -252,6 +244,67 @@ __device__ __forceinline__ void win_grow(int &o, int &w, int n) {
   }
 }
 
+// The LDS window of a block's run of particles a .. b (sorted by cell): nodes
+// o .. o + w - 1 per axis, w <= n, at most cap nodes.  The cells of a sorted
+// run go from its first to its last particle's cell in z-major order, so its
+// nodes fit a box one row segment long (one row) or whole rows wide (several
+// rows of one layer), two layers deep; the rest of the capacity becomes
+// margins in z, y and x for the particles that have drifted since the sort.
+// A run that spans layers, or whose end points are out of order after a long
+// drift, gets the fixed box fx x fy x fz around its middle particle.
+__device__ __forceinline__ void run_window(const State64 &s, int64_t a, int64_t b, const Grid &g,
+                                           int cap, int fx, int fy, int fz, int &wx, int &wy,
+                                           int &wz, int &ox, int &oy, int &oz) {
+  auto cell = [&](double v, int n) { return min(max((int)v, 0), n - 1); };
+  const int ia = cell(s.x[a], g.nx), ja = cell(s.y[a], g.ny), ka = cell(s.z[a], g.nz);
+  const int ib = cell(s.x[b], g.nx), jb = cell(s.y[b], g.ny), kb = cell(s.z[b], g.nz);
+  bool boxed = ka == kb && (jb > ja || (jb == ja && ib >= ia));
+  if (boxed) {
+    const bool row = jb == ja;
+    ox = row ? ia : 0;
+    wx = row ? min(ib - ia + 2, g.nx) : g.nx;
+    oy = ja;
+    wy = min(jb - ja + 2, g.ny);
+    oz = ka;
+    wz = min(2, g.nz);
+    if (wx == g.nx) ox = 0;
+    if (wy == g.ny) oy = 0;
+    if (wz == g.nz) oz = 0;
+    boxed = wx * wy * wz <= cap;
+  }
+  if (boxed) {  // margins while they fit: z, y, x in turn
+    int o[3] = {ox, oy, oz}, w[3] = {wx, wy, wz};
+    const int n[3] = {g.nx, g.ny, g.nz};
+    for (int d = 2, stuck = 0; stuck < 3; d = d == 0 ? 2 : d - 1) {
+      int o2 = o[d], w2 = w[d];
+      bool grew = false;
+      if (w2 < n[d]) {
+        win_grow(o2, w2, n[d]);
+        if ((int64_t)w2 * w[(d + 1) % 3] * w[(d + 2) % 3] <= cap) {
+          o[d] = o2;
+          w[d] = w2;
+          grew = true;
+        }
+      }
+      stuck = grew ? 0 : stuck + 1;
+    }
+    ox = o[0];
+    oy = o[1];
+    oz = o[2];
+    wx = w[0];
+    wy = w[1];
+    wz = w[2];
+  } else {  // the fixed box around the middle particle
+    wx = min(fx, g.nx);
+    wy = min(fy, g.ny);
+    wz = min(fz, g.nz);
+    const int64_t mid = a + (b - a) / 2;
+    ox = wrapn(cell(s.x[mid], g.nx) - wx / 2 + g.nx, g.nx);
+    oy = wrapn(cell(s.y[mid], g.ny) - wy / 2 + g.ny, g.ny);
+    oz = wrapn(cell(s.z[mid], g.nz) - wz / 2 + g.nz, g.nz);
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_deposit64(State64 s, SpeciesTab sp, Grid g,
                                                       double *__restrict__ rho, int64_t ncell,
                                                       int *flag) {
@@ -264,57 +317,7 @@ __global__ __launch_bounds__(kBlock) void k_deposit64(State64 s, SpeciesTab sp, 
   const int64_t off = sp.off[is];
   // window: nodes o .. o + w - 1 per axis (w <= n, so no node appears twice)
   int wx, wy, wz, ox, oy, oz;
-  {
-    auto cell = [&](double v, int n) { return min(max((int)v, 0), n - 1); };
-    const int64_t a = off + beg, b = off + end - 1;
-    const int ia = cell(s.x[a], g.nx), ja = cell(s.y[a], g.ny), ka = cell(s.z[a], g.nz);
-    const int ib = cell(s.x[b], g.nx), jb = cell(s.y[b], g.ny), kb = cell(s.z[b], g.nz);
-    bool boxed = ka == kb && (jb > ja || (jb == ja && ib >= ia));
-    if (boxed) {
-      const bool row = jb == ja;
-      ox = row ? ia : 0;
-      wx = row ? min(ib - ia + 2, g.nx) : g.nx;
-      oy = ja;
-      wy = min(jb - ja + 2, g.ny);
-      oz = ka;
-      wz = min(2, g.nz);
-      if (wx == g.nx) ox = 0;
-      if (wy == g.ny) oy = 0;
-      if (wz == g.nz) oz = 0;
-      boxed = wx * wy * wz <= kWinCap;
-    }
-    if (boxed) {  // margins while they fit: z, y, x in turn
-      int o[3] = {ox, oy, oz}, w[3] = {wx, wy, wz};
-      const int n[3] = {g.nx, g.ny, g.nz};
-      for (int d = 2, stuck = 0; stuck < 3; d = d == 0 ? 2 : d - 1) {
-        int o2 = o[d], w2 = w[d];
-        bool grew = false;
-        if (w2 < n[d]) {
-          win_grow(o2, w2, n[d]);
-          if ((int64_t)w2 * w[(d + 1) % 3] * w[(d + 2) % 3] <= kWinCap) {
-            o[d] = o2;
-            w[d] = w2;
-            grew = true;
-          }
-        }
-        stuck = grew ? 0 : stuck + 1;
-      }
-      ox = o[0];
-      oy = o[1];
-      oz = o[2];
-      wx = w[0];
-      wy = w[1];
-      wz = w[2];
-    } else {  // the fixed box around the middle particle
-      wx = min(kWinX, g.nx);
-      wy = min(kWinY, g.ny);
-      wz = min(kWinZ, g.nz);
-      const int64_t mid = off + min(beg + (int64_t)kBlock * kDepPer / 2, end - 1);
-      ox = wrapn(cell(s.x[mid], g.nx) - wx / 2 + g.nx, g.nx);
-      oy = wrapn(cell(s.y[mid], g.ny) - wy / 2 + g.ny, g.ny);
-      oz = wrapn(cell(s.z[mid], g.nz) - wz / 2 + g.nz, g.nz);
-    }
-  }
+  run_window(s, off + beg, off + end - 1, g, kWinCap, kWinX, kWinY, kWinZ, wx, wy, wz, ox, oy, oz);
   for (int q = threadIdx.x; q < wx * wy * wz; q += kBlock) win[q] = 0ull;
   __syncthreads();
   const double qc = sp.q[is];
@@ -397,15 +400,16 @@ __device__ __forceinline__ Relay relay_axis(double xnew, double lb, double cb, i
 }
 
 // The zigzag sums of a block's particles go to an LDS window (three
-// components, kEmWinX x kEmWinY x kEmWinZ nodes placed around the block's
-// middle particle: after the cell sort a block's particles share a few rows
-// of cells) as int64 fixed point at 2^48 per unit, and reach the global FP64
-// sums once per window node.  A contribution is |vv w| <= 4 in the window
-// (larger fluxes go straight to the global sums), so a node's window sum of
-// at most 2 kBlock kEmPer terms stays below 2^61; the rounding of each term
-// (2^-49) is far below the FP64 summation round-off of the reference's order.
+// components of kEmCap nodes each, shaped from the block's sorted run like the
+// charge window, or a fixed box around its middle particle) as int64 fixed
+// point at 2^48 per unit, and reach the global FP64 sums once per window
+// node.  A contribution is |vv w| <= 4 in the window (larger fluxes go
+// straight to the global sums), so a node's window sum of at most
+// 2 kBlock kEmPer terms stays below 2^61; the rounding of each term (2^-49)
+// is far below the FP64 summation round-off of the reference's order.
 constexpr int kEmPer = 4;
-constexpr int kEmWinX = 40, kEmWinY = 6, kEmWinZ = 5;  // 28.8 KB of int64
+constexpr int kEmCap = 1600;                           // 3 x 12.8 KB of int64
+constexpr int kEmWinX = 40, kEmWinY = 8, kEmWinZ = 5;  // the fixed box
 constexpr double kEmFix = 281474976710656.0;          // 2^48
 
 __device__ __forceinline__ long long em_fix(double v) { return __double2ll_rn(v * kEmFix); }
@@ -414,7 +418,7 @@ __global__ __launch_bounds__(kBlock) void k_push_em64(State64 s, SpeciesTab sp,
                                                       const double *__restrict__ node, Grid g,
                                                       int ustep, double *__restrict__ jacc,
                                                       int64_t ncell, int *flag) {
-  constexpr int WN = kEmWinX * kEmWinY * kEmWinZ;
+  constexpr int WN = kEmCap;
   __shared__ unsigned long long win[3 * WN];
   const int is = blockIdx.y;
   const int64_t beg = (int64_t)blockIdx.x * kBlock * kEmPer;
@@ -423,12 +427,11 @@ __global__ __launch_bounds__(kBlock) void k_push_em64(State64 s, SpeciesTab sp,
   const int64_t end = min(beg + (int64_t)kBlock * kEmPer, cnt);
   const int64_t off = sp.off[is];
   // window: nodes o .. o + w - 1 per axis (w <= n, so no node appears twice)
-  const int wx = min(kEmWinX, g.nx), wy = min(kEmWinY, g.ny), wz = min(kEmWinZ, g.nz);
-  const int64_t mid = off + min(beg + (int64_t)kBlock * kEmPer / 2, end - 1);
-  const int ox = wrapn(min((int)s.x[mid], g.nx - 1) - wx / 2 + g.nx, g.nx);
-  const int oy = wrapn(min((int)s.y[mid], g.ny - 1) - wy / 2 + g.ny, g.ny);
-  const int oz = wrapn(min((int)s.z[mid], g.nz - 1) - wz / 2 + g.nz, g.nz);
-  for (int q = threadIdx.x; q < 3 * WN; q += kBlock) win[q] = 0ull;
+  int wx, wy, wz, ox, oy, oz;
+  run_window(s, off + beg, off + end - 1, g, WN, kEmWinX, kEmWinY, kEmWinZ, wx, wy, wz, ox, oy,
+             oz);
+  const int wn = wx * wy * wz;  // nodes per component
+  for (int q = threadIdx.x; q < 3 * wn; q += kBlock) win[q] = 0ull;
   __syncthreads();
   double gustep = (double)ustep;
   double rustep = 1.0 / gustep;
@@ -483,25 +486,24 @@ __global__ __launch_bounds__(kBlock) void k_push_em64(State64 s, SpeciesTab sp,
     auto seg = [&](int i, int j, int k, double vvx, double vvy, double vvz, const double wx4[4],
                    const double wy4[4], const double wz4[4]) {
       const int li = wrapn(i - ox, g.nx), lj = wrapn(j - oy, g.ny), lk = wrapn(k - oz, g.nz);
-      const bool in_win = li <= wx - 2 && lj <= wy - 2 && lk <= wz - 2 && fabs(vvx) <= 4.0 &&
-                          fabs(vvy) <= 4.0 && fabs(vvz) <= 4.0;
+      int li1, lj1, lk1;
+      const bool in_win = li < wx && lj < wy && lk < wz && win_next(li, wx, g.nx, li1) &&
+                          win_next(lj, wy, g.ny, lj1) && win_next(lk, wz, g.nz, lk1) &&
+                          fabs(vvx) <= 4.0 && fabs(vvy) <= 4.0 && fabs(vvz) <= 4.0;
       if (in_win) {
-        unsigned long long *p = win + (lk * wy + lj) * wx + li;
-        const int X = 1, Y = wx, Z = wx * wy;
-        atomicAdd(p, (unsigned long long)em_fix(vvx * wx4[0]));
-        atomicAdd(p + Y, (unsigned long long)em_fix(vvx * wx4[1]));
-        atomicAdd(p + Z, (unsigned long long)em_fix(vvx * wx4[2]));
-        atomicAdd(p + Y + Z, (unsigned long long)em_fix(vvx * wx4[3]));
-        p += WN;
-        atomicAdd(p, (unsigned long long)em_fix(vvy * wy4[0]));
-        atomicAdd(p + X, (unsigned long long)em_fix(vvy * wy4[1]));
-        atomicAdd(p + Z, (unsigned long long)em_fix(vvy * wy4[2]));
-        atomicAdd(p + X + Z, (unsigned long long)em_fix(vvy * wy4[3]));
-        p += WN;
-        atomicAdd(p, (unsigned long long)em_fix(vvz * wz4[0]));
-        atomicAdd(p + X, (unsigned long long)em_fix(vvz * wz4[1]));
-        atomicAdd(p + Y, (unsigned long long)em_fix(vvz * wz4[2]));
-        atomicAdd(p + X + Y, (unsigned long long)em_fix(vvz * wz4[3]));
+        auto at = [&](int c, int a, int b, int d) { return win + c * wn + (d * wy + b) * wx + a; };
+        atomicAdd(at(0, li, lj, lk), (unsigned long long)em_fix(vvx * wx4[0]));
+        atomicAdd(at(0, li, lj1, lk), (unsigned long long)em_fix(vvx * wx4[1]));
+        atomicAdd(at(0, li, lj, lk1), (unsigned long long)em_fix(vvx * wx4[2]));
+        atomicAdd(at(0, li, lj1, lk1), (unsigned long long)em_fix(vvx * wx4[3]));
+        atomicAdd(at(1, li, lj, lk), (unsigned long long)em_fix(vvy * wy4[0]));
+        atomicAdd(at(1, li1, lj, lk), (unsigned long long)em_fix(vvy * wy4[1]));
+        atomicAdd(at(1, li, lj, lk1), (unsigned long long)em_fix(vvy * wy4[2]));
+        atomicAdd(at(1, li1, lj, lk1), (unsigned long long)em_fix(vvy * wy4[3]));
+        atomicAdd(at(2, li, lj, lk), (unsigned long long)em_fix(vvz * wz4[0]));
+        atomicAdd(at(2, li1, lj, lk), (unsigned long long)em_fix(vvz * wz4[1]));
+        atomicAdd(at(2, li, lj1, lk), (unsigned long long)em_fix(vvz * wz4[2]));
+        atomicAdd(at(2, li1, lj1, lk), (unsigned long long)em_fix(vvz * wz4[3]));
       } else {
         const int i1 = i + 1, j1 = j + 1, k1 = k + 1;
         atomicAdd(jx + cidx(g, i, j, k), vvx * wx4[0]);
@@ -526,10 +528,10 @@ __global__ __launch_bounds__(kBlock) void k_push_em64(State64 s, SpeciesTab sp,
     seg(rx.ia, ry.ia, rz.ia, vvx2, vvy2, vvz2, ax2, ay2, az2);
   }
   __syncthreads();
-  for (int q = threadIdx.x; q < 3 * WN; q += kBlock) {
+  for (int q = threadIdx.x; q < 3 * wn; q += kBlock) {
     const long long v = (long long)win[q];
     if (v != 0) {
-      const int d = q / WN, r = q - d * WN;
+      const int d = q / wn, r = q - d * wn;
       const int li = r % wx, lj = (r / wx) % wy, lk = r / (wx * wy);
       double *jd = d == 0 ? jx : (d == 1 ? jy : jz);
       atomicAdd(jd + cidx(g, ox + li, oy + lj, oz + lk), (double)v * (1.0 / kEmFix));
