@@ -15,6 +15,9 @@ static thread_local std::string g_last_error;
 
 int set_error(int code, const std::string &msg) {
   g_last_error = msg;
+  // a failed allocation also sets HIP's last error, which the launch checks
+  // of later calls read: the failure is reported here, so clear it
+  if (code == EMG_ERR_NOMEM) (void)hipGetLastError();
   return code;
 }
 

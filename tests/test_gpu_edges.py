@@ -1,7 +1,8 @@
 """Edge cases of the particle path on the GPU against the oracle: an empty
 species, a grid of one tile per axis (every neighbouring tile is the tile
-itself through the periodic wrap), particles on the box faces, and the
-capacity limit of the FP32 form (refused before anything is allocated)."""
+itself through the periodic wrap), particles on the box faces, the capacity
+limit of the FP32 form (refused before anything is allocated), and a device
+allocation that fails (an error code, and the library keeps working)."""
 import numpy as np
 import pytest
 
@@ -122,3 +123,22 @@ def test_fp32_capacity_limit_is_refused():
     with pytest.raises(emg.EmgError) as e:
         emg.Context((64, 64, 64), Q, QM, RM, (1 << 29, 1 << 20), precision=emg.FP32_FIXED)
     assert "2^29" in str(e.value)
+
+
+def test_failed_device_allocation_returns_an_error():
+    """A context far larger than the GPU's memory (2^37 FP64 particles, over
+    6 TB) fails with EMG_ERR_NOMEM, not an abort; contexts created after it
+    push and deposit normally (the failed allocation leaves no error behind
+    for the later launch checks)."""
+    with pytest.raises(emg.EmgError) as e:
+        emg.Context((64, 64, 64), Q, QM, RM, (1 << 37, 1), precision=emg.PLAIN_FP64)
+    assert e.value.code == -3, str(e.value)   # EMG_ERR_NOMEM
+    n = (8, 8, 8)
+    counts = (4000, 1000)
+    p = particles(n, counts, (0.1, 0.01), 9)
+    eb = eb_random(n, 4)
+    res = run_both(n, counts, p, eb, 2, tile=4)
+    o, rhodg = oracle_run(n, counts, res[emg.PLAIN_FP64][2], eb, 2)
+    check(n, counts, res[emg.PLAIN_FP64], o, rhodg, True)
+    o, rhodg = oracle_run(n, counts, res[emg.FP32_FIXED][2], eb, 2)
+    check(n, counts, res[emg.FP32_FIXED], o, rhodg, False)
