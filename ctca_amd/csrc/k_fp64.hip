@@ -245,32 +245,56 @@ __device__ __forceinline__ void win_grow(int &o, int &w, int n) {
 }
 
 // The LDS window of a block's run of particles a .. b (sorted by cell): nodes
-// o .. o + w - 1 per axis, w <= n, at most cap nodes.  The cells of a sorted
-// run go from its first to its last particle's cell in z-major order, so its
-// nodes fit a box one row segment long (one row) or whole rows wide (several
-// rows of one layer), two layers deep; the rest of the capacity becomes
-// margins in z, y and x for the particles that have drifted since the sort.
-// A run that spans layers, or whose end points are out of order after a long
-// drift, gets the fixed box fx x fy x fz around its middle particle.
+// o .. o + w - 1 per axis (periodic), w <= n, at most cap nodes.  The cells of
+// a sorted run go from its first to its last particle's cell in z-major order
+// (rows of the y-z plane one after another), so its nodes fit a box of its
+// rows plus one, its layers plus one, and in x the row segment it covers (or,
+// for a run that passes into the next row, the periodic range from its first
+// cell to its last) or whole rows; the rest of the capacity becomes margins
+// in z, y and x for the particles that have drifted since the sort.  A run of
+// more rows than a layer holds, or a box over the capacity, gets the fixed
+// box fx x fy x fz around its middle particle.
 __device__ __forceinline__ void run_window(const State64 &s, int64_t a, int64_t b, const Grid &g,
                                            int cap, int fx, int fy, int fz, int &wx, int &wy,
                                            int &wz, int &ox, int &oy, int &oz) {
   auto cell = [&](double v, int n) { return min(max((int)v, 0), n - 1); };
   const int ia = cell(s.x[a], g.nx), ja = cell(s.y[a], g.ny), ka = cell(s.z[a], g.nz);
   const int ib = cell(s.x[b], g.nx), jb = cell(s.y[b], g.ny), kb = cell(s.z[b], g.nz);
-  bool boxed = ka == kb && (jb > ja || (jb == ja && ib >= ia));
+  // the run's rows, as consecutive rows of the periodic y-z plane
+  const int64_t ra = (int64_t)ka * g.ny + ja, rb = (int64_t)kb * g.ny + jb;
+  const int64_t nrow = (rb - ra + (int64_t)g.ny * g.nz) % ((int64_t)g.ny * g.nz) + 1;
+  bool boxed = nrow <= g.ny;
   if (boxed) {
-    const bool row = jb == ja;
-    ox = row ? ia : 0;
-    wx = row ? min(ib - ia + 2, g.nx) : g.nx;
+    // x: the row segment ia..ib, or (a run into the next row) ia..nx-1 and
+    // 0..ib taken as one periodic range; whole rows for longer runs
+    if (nrow == 1 && ib >= ia) {
+      ox = ia;
+      wx = ib - ia + 2;
+    } else if (nrow == 2 && ib < ia) {
+      ox = ia;
+      wx = ib + g.nx - ia + 2;
+    } else {
+      ox = 0;
+      wx = g.nx;
+    }
+    // y: rows ja .. jb (periodic); z: layers ka .. kb
     oy = ja;
-    wy = min(jb - ja + 2, g.ny);
+    wy = (int)min<int64_t>(nrow + 1, g.ny);
     oz = ka;
-    wz = min(2, g.nz);
-    if (wx == g.nx) ox = 0;
-    if (wy == g.ny) oy = 0;
-    if (wz == g.nz) oz = 0;
-    boxed = wx * wy * wz <= cap;
+    wz = (kb - ka + g.nz) % g.nz + 2;
+    if (wx >= g.nx) {
+      ox = 0;
+      wx = g.nx;
+    }
+    if (wy >= g.ny) {
+      oy = 0;
+      wy = g.ny;
+    }
+    if (wz >= g.nz) {
+      oz = 0;
+      wz = g.nz;
+    }
+    boxed = (int64_t)wx * wy * wz <= cap;
   }
   if (boxed) {  // margins while they fit: z, y, x in turn
     int o[3] = {ox, oy, oz}, w[3] = {wx, wy, wz};
