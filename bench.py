@@ -506,8 +506,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the particle path runs on one MI355X that owns the "
-              "whole domain; sharding over several GPUs is a design in DESIGN.md, not a "
-              "measured mode.", file=sys.stderr)
+              "whole domain; sharding over several GPUs is designed (z slabs with particle "
+              "migration between neighbours) but not a measured mode.", file=sys.stderr)
         sys.exit(2)
 
     # torch must take the device before libemgpu does: in the other order

@@ -76,8 +76,8 @@ def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
-            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() "
-                               "(python -c 'import __graft_entry__ as g; g.build()')")
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with "
+                               "'python -m ctca_amd.build'")
         L = C.CDLL(LIB_PATH)
         i32p = C.POINTER(C.c_int32)
         vp = C.c_void_p
