@@ -11,7 +11,14 @@ fold) -> bfield(1/2) -> efield(-2 aj) -> charge (diagnostic).
 
 The particle work runs in libemgpu through ctca_amd.emg; the field solve is
 the host side (ctca_amd.fieldsolve).
+
+Job continuation follows &jobcon jobnum(1:2): jobnum(1) = 1 starts from
+./SNAPSHOT0/esdat0000.h5 (inipcl.F90:359-426) and takes the half push
+positn(1, 1) (esses.F90:198-201, a continued job only); jobnum(2) = 1 writes
+./SNAPSHOT1/esdat0000.h5 at the end (esses_finalize, esses.F90:481-494).
 """
+import os
+
 import numpy as np
 
 from . import emg, fieldsolve, units
@@ -21,16 +28,27 @@ class ESRun:
     """An electrostatic run of a workload on the GPU path."""
 
     def __init__(self, w: units.Workload, precision=emg.PLAIN_FP64, init="host",
-                 count_scale=1.0, tile=0, particles=None):
+                 count_scale=1.0, tile=0, particles=None, jobnum=(0, 0), root="."):
         self.w = w
         self.su = su = units.to_sunits(w)
         self.n = tuple(w.n)
+        self.jobnum = (int(jobnum[0]), int(jobnum[1]))
+        if self.jobnum[0] not in (0, 1) or self.jobnum[1] not in (0, 1):
+            raise ValueError("jobnum(1) must be 0 or 1 (ES), jobnum(2) 0 or 1")
+        self.root = root
         mobile = [k for k, s in enumerate(w.species) if s.mobile]
         self.q = su.q[mobile]
         npin = np.array([int(round(w.species[k].npin * count_scale)) for k in mobile])
         self.ctx = emg.Context(self.n, su.q[mobile], su.qm[mobile], su.rm[mobile], npin,
                                precision=precision, e0=su.e0, b0=su.b0, tile=tile)
-        if particles is not None:
+        self.rhobk = fieldsolve.background(su.q, su.npin, self.n)
+        if self.jobnum[0] == 1:
+            from . import snapshot
+            snap = snapshot.load(self.ctx, snapshot.snapshot_file(
+                os.path.join(root, snapshot.LOAD_DIR), 0))
+            self.ctx.set_background(None)   # the host solve adds the background itself
+            self.rhobk = np.ascontiguousarray(snap.rhobk[:-1, :-1, :-1])
+        elif particles is not None:
             p, counts = particles
             self.ctx.upload(p, counts)
         elif init == "host":
@@ -39,7 +57,6 @@ class ESRun:
             self.ctx.upload(p, npin)
         else:
             self.ctx.init_device(units.device_species(w, su, count_scale), w.seed)
-        self.rhobk = fieldsolve.background(su.q, su.npin, self.n)
         self.istep = 0
         self.history = []
 
@@ -56,9 +73,10 @@ class ESRun:
         return rho + self.rhobk
 
     def initialize(self):
-        """esses_initialize from chgntr on: positn(1,1) half push, charge,
-        Poisson, E, energy (esses.F90:190-263)."""
-        self.ctx.positn(1)
+        """esses_initialize from chgntr on: the half push positn(1,1) of a
+        continued job, charge, Poisson, E, energy (esses.F90:190-263)."""
+        if self.jobnum[0] == 1:
+            self.ctx.positn(1)
         rho = self.charge()
         self.eb = self.solve_fields(rho)
         self.rho = rho
@@ -83,6 +101,16 @@ class ESRun:
     def run(self, nstep):
         for k in range(1, nstep + 1):
             self.step(2 if k != nstep else 1)
+
+    def finalize(self):
+        """esses_finalize: with jobnum(2) = 1 the particles and the background
+        go to ./SNAPSHOT1/esdat0000.h5 (save_esdat, snap.F90:203-356)."""
+        if self.jobnum[1] == 1:
+            from . import snapshot
+            rb = np.broadcast_to(np.asarray(self.rhobk, dtype=np.float64),
+                                 (self.n[2], self.n[1], self.n[0]))
+            snapshot.save(self.ctx, snapshot.snapshot_file(
+                os.path.join(self.root, snapshot.SAVE_DIR), 0), rhobk=rb)
 
     def close(self):
         self.ctx.close()

@@ -143,10 +143,9 @@ def test_es32_loop_fp64_matches_oracle():
     counts = su.npin
     from ctca_amd import driver
     run = driver.ESRun(w, particles=(p0, counts))
-    run.initialize()
+    run.initialize()           # a fresh job: no positn (esses.F90:198-201)
     o = p0.copy()
     sl = np.array(n, dtype=float)
-    oracle.positn(o, 1, sl)
 
     def o_rho(o):
         r = oracle.charge(o, counts, su.q, n)

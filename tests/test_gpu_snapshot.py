@@ -116,3 +116,33 @@ def test_resume_equals_uninterrupted(precision, tmp_path):
             assert np.minimum(d, L - d).max() <= 1e-12 * L, f
         for f in ("vx", "vy", "vz"):
             assert np.abs(whole[f] - resumed[f]).max() <= 1e-12, f
+
+
+def test_driver_job_continuation(tmp_path):
+    """The host loop's jobnum: a job with jobnum = (0, 1) runs and writes
+    SNAPSHOT1 at its end; moved to SNAPSHOT0, a job with jobnum = (1, 0)
+    starts from it with the half push positn(1, 1) of a continued job
+    (esses.F90:198-201): its particles are the first job's last ones moved by
+    the oracle's positn, bit for bit, and its background is the file's."""
+    import os
+    import oracle
+    from ctca_amd import driver, units
+    w = units.es32()
+    su = units.to_sunits(w)
+    a = driver.ESRun(w, jobnum=(0, 1), root=str(tmp_path))
+    a.initialize()
+    a.run(3)
+    last, counts = a.ctx.download()
+    a.finalize()
+    a.close()
+    os.rename(tmp_path / snapshot.SAVE_DIR, tmp_path / snapshot.LOAD_DIR)
+    b = driver.ESRun(w, jobnum=(1, 0), root=str(tmp_path))
+    assert np.all(b.rhobk == a.rhobk)
+    b.initialize()
+    got, cnt = b.ctx.download()
+    b.close()
+    assert list(cnt) == list(counts)
+    ref = last.copy()
+    oracle.positn(ref, 1, np.array(w.n, dtype=float))
+    for f in ("x", "y", "z", "vx", "vy", "vz"):
+        assert np.array_equal(got[f], ref[f]), f
