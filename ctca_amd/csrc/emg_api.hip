@@ -23,6 +23,7 @@ int set_error(int code, const std::string &msg) {
 
 int hip_check(hipError_t e, const char *what) {
   if (e == hipSuccess) return EMG_OK;
+  (void)hipGetLastError();  // reported now, not again by a later launch check
   return set_error(e == hipErrorOutOfMemory ? EMG_ERR_NOMEM : EMG_ERR_HIP,
                    std::string(what) + ": " + hipGetErrorString(e));
 }
