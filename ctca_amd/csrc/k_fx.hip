@@ -40,7 +40,13 @@ namespace emg {
 namespace {
 
 constexpr int kSpillBlocksPerSpecies = 64;
-constexpr int kOutbox = 512;  // movers staged per bucket (LDS and global outbox)
+// Movers staged per bucket: the global outboxes hold kObStride records per
+// bucket; a step kernel stages up to ob_cap of them in LDS (the current's
+// three LDS tiles leave room for 512 at two blocks per CU; the charge kernel
+// takes 1024, above the ~490 movers per bucket of a 268M-particle step)
+constexpr int kObStride = 1024;
+template <int DEP>
+constexpr int ob_cap() { return DEP == 2 ? 512 : 1024; }
 constexpr int kExtra = 16;    // slots per bucket for direct arrivals
 // bucket capacities are multiples of 32 slots, so every bucket starts on a
 // 128-byte line of each state array and every wave's load of a round covers
@@ -597,7 +603,7 @@ __device__ __forceinline__ void fx_one(const FxArgs &a, int s, bool act, const P
     obase = __builtin_amdgcn_readlane(obase, leader);
     int k = obase + __builtin_amdgcn_mbcnt_hi((uint32_t)(mv >> 32),
                                               __builtin_amdgcn_mbcnt_lo((uint32_t)mv, 0));
-    const bool boxed = mover && lds_ok && k < kOutbox;
+    const bool boxed = mover && lds_ok && k < ob_cap<DEP>();
     if (boxed) {
       FxRec r;
       r.cell = cell;
@@ -678,7 +684,7 @@ __device__ __forceinline__ void pull_table(const FxArgs &a, int b, int64_t *seg_
       int nbk = neighbour_bucket(b, 26 - c, a.ntx, a.nty, a.ntz, a.ntiles);
       int lo = a.pre_in[(int64_t)nbk * kPre + c], hi = a.pre_in[(int64_t)nbk * kPre + c + 1];
       n_c = hi - lo;
-      src = (int64_t)nbk * kOutbox + lo;
+      src = (int64_t)nbk * kObStride + lo;
     }
     int v = n_c;
 #pragma unroll
@@ -709,13 +715,15 @@ template <int T, bool PUSH, int DEP, bool PID>
 __global__ __launch_bounds__(T == 8 ? 512 : 256) void k_fx_step(FxArgs a) {
   constexpr int NT = T == 8 ? 512 : 256;  // threads: T^3 cells x 64 particles per cell
   constexpr int N1 = T + 1, N3 = T + 3;
-  constexpr int kPer = kOutbox / NT;
+  constexpr int kOb = ob_cap<DEP>();
+  constexpr int kPer = kOb / NT;
+  static_assert(kOb % NT == 0, "the outbox is written kPer records per thread");
   __shared__ float4 fA[PUSH ? N1 * N1 * N1 : 1];
   __shared__ float2 fB[PUSH ? N1 * N1 * N1 : 1];
   constexpr int kTileNodes = DEP == 1 ? N3 * N3 * N3 : (DEP == 2 ? 3 * N3 * N3 * N3 : 1);
   __shared__ unsigned long long rh[kTileNodes];  // charge tile, or the three current tiles
-  __shared__ FxRec outbox[kOutbox];
-  __shared__ unsigned char ocode[kOutbox];
+  __shared__ FxRec outbox[kOb];
+  __shared__ unsigned char ocode[kOb];
   __shared__ int64_t seg_src[kCodes];
   __shared__ int seg_pre[kPre];
   __shared__ int n_stay, n_out;
@@ -846,7 +854,7 @@ __global__ __launch_bounds__(T == 8 ? 512 : 256) void k_fx_step(FxArgs a) {
   // ---- write the outbox grouped by direction
   __syncthreads();
   if (b >= 0) {
-    const int nout = min(n_out, kOutbox);
+    const int nout = min(n_out, kOb);
     const int t = b - s * a.ntiles;
     int cq[kPer], rq[kPer];
 #pragma unroll
@@ -883,7 +891,7 @@ __global__ __launch_bounds__(T == 8 ? 512 : 256) void k_fx_step(FxArgs a) {
       if (q < nout) {
         const FxRec r = outbox[q];
         if (cq[k] < kFar) {
-          a.ob_out[(int64_t)b * kOutbox + opre[cq[k]] + rq[k]] = r;
+          a.ob_out[(int64_t)b * kObStride + opre[cq[k]] + rq[k]] = r;
         } else {
           place_direct(a, s * a.ntiles + tile_of_cell<T>(r.cell, a.ntx, a.nty), r.cell, r.pid,
                        r.fx, r.fy, r.fz, r.vx, r.vy, r.vz);
@@ -1150,7 +1158,7 @@ __device__ __forceinline__ void range_load(const RangeDesc &d, const StateFx &in
   } else if (r < d.nrb + d.nspec) {
     load_pcl(in, d.seg_start[r - d.nrb] + i, p);
   } else {
-    load_rec(d.ob, (int64_t)(r - d.nrb - d.nspec) * kOutbox + i, p);
+    load_rec(d.ob, (int64_t)(r - d.nrb - d.nspec) * kObStride + i, p);
   }
 }
 
@@ -1450,7 +1458,7 @@ int fx_alloc(emg_ctx *c, int64_t ntotal) {
         hipMalloc(&B.spill[k], sizeof(int32_t) * EMG_MAX_SPECIES) != hipSuccess ||
         hipMalloc(&B.pre[k], sizeof(int32_t) * B.nb * kPre) != hipSuccess)
       return set_error(EMG_ERR_NOMEM, "bucket tables");
-    if (hipMalloc(&B.ob[k], sizeof(FxRec) * B.nb * kOutbox) != hipSuccess)
+    if (hipMalloc(&B.ob[k], sizeof(FxRec) * B.nb * kObStride) != hipSuccess)
       return set_error(EMG_ERR_NOMEM, "bucket outboxes");
     launch_zero(B.count[k], sizeof(int32_t) * B.nb, c->stream);
     launch_zero(B.nst[k], sizeof(int32_t) * B.nb, c->stream);
