@@ -792,7 +792,7 @@ static int read_grid(emg_ctx *c, double *host, const int32_t lo[3], const int32_
   int rc = dalloc(&d_out, (size_t)n, "read-out");
   if (rc) return rc;
   if (current)
-    launch_read_current(c, d_out, l, h);
+    launch_read_current(c, species, d_out, l, h);
   else
     launch_read_rho(c, species, d_out, l, h);
   rc = check_flags(c);
@@ -860,6 +860,13 @@ int emg_get_current(emg_ctx *c, double *aj, const int32_t lo[3], const int32_t h
   CTX_CHECK(c);
   if (!c->j_valid) return set_error(EMG_ERR_STATE, "no current deposited");
   return read_grid(c, aj, lo, hi, 3, -1, true);
+}
+
+int emg_get_ajdg(emg_ctx *c, int32_t is, double *aj, const int32_t lo[3], const int32_t hi[3]) {
+  CTX_CHECK(c);
+  if (!c->j_valid) return set_error(EMG_ERR_STATE, "no current deposited");
+  if (is < 0 || is >= c->sp.nspec) return set_error(EMG_ERR_ARG, "species index");
+  return read_grid(c, aj, lo, hi, 3, is, true);
 }
 
 // ---- adjoint window (FP64 form)

@@ -209,7 +209,7 @@ void es_solver_free(emg_ctx *c);
 void launch_relocate32(emg_ctx *c);  // node64-independent: from eb_dev
 void launch_energy64(emg_ctx *c);
 void launch_read_rho(emg_ctx *c, int species, double *dst_dev, const int lo[3], const int hi[3]);
-void launch_read_current(emg_ctx *c, double *dst_dev, const int lo[3], const int hi[3]);
+void launch_read_current(emg_ctx *c, int species, double *dst_dev, const int lo[3], const int hi[3]);
 void launch_zero(void *p, size_t bytes, hipStream_t s);
 int sort64(emg_ctx *c);
 int init_device(emg_ctx *c, const emg_device_species *spec, int nspec, uint64_t seed);

@@ -204,6 +204,7 @@ __global__ __launch_bounds__(kBlock) void k_read_current(ReadArgs r, const void 
   for (int d = 0; d < 3; d++) {
     double v = 0.0;
     for (int s = 0; s < r.nspec; s++) {
+      if (r.species >= 0 && s != r.species) continue;
       int64_t q = ((int64_t)s * 3 + d) * r.ncell + c;
       double js = FIX ? (double)(long long)((const unsigned long long *)jacc)[q] * (1.0 / kJScale)
                       : ((const double *)jacc)[q];
@@ -410,8 +411,8 @@ void launch_read_rho(emg_ctx *c, int species, double *dst, const int lo[3], cons
     hipLaunchKernelGGL(k_read_rhofx, dim3(grid), dim3(kBlock), 0, c->stream, r, c->rhofx, dst);
 }
 
-void launch_read_current(emg_ctx *c, double *dst, const int lo[3], const int hi[3]) {
-  ReadArgs r = read_args(c, -1, lo, hi);
+void launch_read_current(emg_ctx *c, int species, double *dst, const int lo[3], const int hi[3]) {
+  ReadArgs r = read_args(c, species, lo, hi);
   int64_t n = (int64_t)r.ext[0] * r.ext[1] * r.ext[2];
   unsigned grid = (unsigned)((n + kBlock - 1) / kBlock);
   if (c->cfg.precision == EMG_PLAIN_FP64)

@@ -38,7 +38,7 @@ EXPORTS = (
     "emg_state_save", "emg_state_load", "emg_adj_reset", "emg_adj_step", "emg_adj_get_field",
     "emg_adj_add_density", "emg_adj_velocity_sums",
     "emg_bfield", "emg_efield", "emg_grdcrt", "emg_get_fields", "emg_field_energy",
-    "emg_solve_es", "emg_get_potential", "emg_set_background",
+    "emg_solve_es", "emg_get_potential", "emg_set_background", "emg_get_ajdg",
 )
 
 
@@ -99,6 +99,7 @@ def lib():
             "emg_energy": [vp, vp],
             "emg_push_em": [vp, C.c_int32],
             "emg_get_current": [vp, vp, i32p, i32p],
+            "emg_get_ajdg": [vp, C.c_int32, vp, i32p, i32p],
             "emg_sort": [vp],
             "emg_init_device": [vp, C.POINTER(DeviceSpecies), C.c_int32, C.c_uint64],
             "emg_host_inipcl": [C.POINTER(HostSpecies), C.c_int32, i32p, C.POINTER(C.c_double),
@@ -332,11 +333,15 @@ class Context:
             _chk(lib().emg_get_rhodg(self.h, int(species), _ptr(out), _i3(lo), _i3(hi)))
         return out
 
-    def current(self, lo=None, hi=None):
-        """aj(3, lo:hi)^3 as numpy [k, j, i, 3]."""
+    def current(self, lo=None, hi=None, species=None):
+        """aj(3, lo:hi)^3 as numpy [k, j, i, 3]; with species, that species'
+        ajdg alone."""
         lo, hi = self._bounds(lo, hi)
         out = np.zeros((hi[2] - lo[2] + 1, hi[1] - lo[1] + 1, hi[0] - lo[0] + 1, 3))
-        _chk(lib().emg_get_current(self.h, _ptr(out), _i3(lo), _i3(hi)))
+        if species is None:
+            _chk(lib().emg_get_current(self.h, _ptr(out), _i3(lo), _i3(hi)))
+        else:
+            _chk(lib().emg_get_ajdg(self.h, int(species), _ptr(out), _i3(lo), _i3(hi)))
         return out
 
     def energy(self):

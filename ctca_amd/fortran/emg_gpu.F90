@@ -172,6 +172,15 @@ module emg_gpu
       integer(c_int32_t), dimension(3), intent(in) :: lo, hi
     end function emg_get_current
 
+    ! ajdg of species is (0-based): q(is) J(is), folded (psolve1.F90:803-817)
+    integer(c_int) function emg_get_ajdg(ctx, is, aj, lo, hi) bind(C, name="emg_get_ajdg")
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: is
+      real(c_double), dimension(*), intent(out) :: aj
+      integer(c_int32_t), dimension(3), intent(in) :: lo, hi
+    end function emg_get_ajdg
+
     integer(c_int) function emg_sort(ctx) bind(C, name="emg_sort")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

@@ -159,6 +159,10 @@ int emg_energy(emg_ctx *ctx, double *engk);
  */
 int emg_push_em(emg_ctx *ctx, int32_t ustep);
 int emg_get_current(emg_ctx *ctx, double *aj, const int32_t lo[3], const int32_t hi[3]);
+/* The current of species is (0-based) alone, q(is) J(is) folded like aj: the
+ * per-species diagnostic ajdg(3 is + 1 : 3 is + 3, ...) of psolve1 with
+ * func = 1 (psolve1.F90:803-817). */
+int emg_get_ajdg(emg_ctx *ctx, int32_t is, double *aj, const int32_t lo[3], const int32_t hi[3]);
 
 /*
  * The EM field update on the library's copy of eb (the array emg_set_fields

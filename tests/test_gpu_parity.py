@@ -547,6 +547,43 @@ def test_push_em_fp64_vs_oracle():
     assert np.abs(J - oj).max() <= 1e-12 * np.abs(oj).max()
 
 
+@pytest.mark.parametrize("precision", [emg.PLAIN_FP64, emg.FP32_FIXED])
+def test_ajdg_per_species_current(precision):
+    """emg_get_ajdg (psolve1 func = 1, psolve1.F90:803-817): each species'
+    q J alone; the species add up to aj, and in the FP64 form each matches the
+    oracle's current with the other species' charge set to zero."""
+    n = (12, 8, 8)
+    counts = (20000, 5000)
+    q, qm, rm = (0.002, -0.003), (1.0, -0.5), (0.002, 0.006)
+    rng = np.random.default_rng(9)
+    eb = periodic_eb(n, rng, 0.02, 0.01)
+    b0 = (0.0, 0.0, 0.02)
+    p = mt_particles(n, counts, (0.1, 0.05), drift=(0.0, 0.1))
+    kw = dict(tile=4, track_pid=True) if precision == emg.FP32_FIXED else {}
+    lo, hi = (-1, -1, -1), tuple(v + 1 for v in n)
+    with ctx_for(n, q, qm, rm, counts, precision=precision, b0=b0, **kw) as ctx:
+        ctx.upload(p, counts)
+        start, _ = ctx.download()
+        ctx.set_fields(eb)
+        ctx.push_em(2)
+        J = ctx.current(lo=lo, hi=hi)
+        Js = [ctx.current(lo=lo, hi=hi, species=k) for k in range(2)]
+        with pytest.raises(emg.EmgError):
+            ctx.current(species=2)
+    scale = np.abs(J).max()
+    assert np.abs(Js[0] + Js[1] - J).max() <= 1e-15 * scale
+    assert np.abs(Js[1]).max() > 0.1 * scale and np.abs(Js[0]).max() > 0.1 * scale
+    if precision == emg.PLAIN_FP64:
+        wrk = oracle.grdcrt(eb, n, (0, 0, 0), b0)
+        for k in range(2):
+            qk = tuple(q[s] if s == k else 0.0 for s in range(2))
+            o = start.copy()
+            aj = oracle.psolve1(o, counts, qm, qk, wrk, n, 2, np.array(n, dtype=float))
+            oracle.fold_current(aj, n)
+            oj = aj[3:n[2] + 6, 3:n[1] + 6, 3:n[0] + 6]
+            assert np.abs(Js[k] - oj).max() <= 1e-12 * np.abs(oj).max(), k
+
+
 @pytest.mark.parametrize("tile,n", [(4, (16, 12, 12)), (4, (12, 8, 8)), (8, (16, 16, 8))])
 def test_push_em_fp32_fixed_vs_fp64_oracle(tile, n):
     """FP32 zigzag push: one step against the oracle's psolve1 from the same
