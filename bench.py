@@ -31,7 +31,10 @@ Further legs, reported inside `config` (each can be switched off):
     (474,679,216 particles), the worst-loaded share of 8 GPUs;
   * energy_drift: the plain ES configuration (32^3, 1,048,576 particles) run
     self-consistently on the device (deposit, field solve, push) in both
-    forms, with the drift of kinetic + field energy.
+    forms, with the drift of kinetic + field energy;
+  * em_energy_drift: the EM two-stream workload at full size with the Yee
+    update on the device, 100 steps in both forms, the same for kinetic +
+    electric + magnetic energy.
 
 Prints one JSON line.  `roofline` is the fused FP32 kernel: algorithmic bytes
 per launch (56 B per particle + 24 B per field node + 8 B per charge node and
@@ -344,6 +347,51 @@ def energy_drift(steps):
     return out
 
 
+def em_energy_drift(steps):
+    """The EM two-stream workload at full size run with the fields on the
+    device (bfield, grdcrt, psolve1 + current, bfield, efield; emses.F90:
+    365-434) for `steps` steps in both forms, from the same seeded state and
+    the E of its initial density: total energy W = kinetic + 0.5 sum E^2 / V
+    + 0.5 cs sum B^2 / V (energy.F90:30-70) after each step, its largest
+    relative change from the first step, and the largest difference of the
+    two forms' histories."""
+    from ctca_amd import emg, units
+    w = units.em_twostream()
+    su = units.to_sunits(w)
+    n = w.n
+    spec = units.device_species(w, su)
+    tcs = 2.0 * su.cs
+    out = {"workload": w.name, "steps": steps, "particles": int(su.npin.sum())}
+    hist = {}
+    eb = None
+    for name, prec in (("fp32", emg.FP32_FIXED), ("fp64", emg.PLAIN_FP64)):
+        kw = dict(tile=auto_tile(n)) if prec == emg.FP32_FIXED else dict(sort_every=FP64_SORT_EVERY)
+        ctx = emg.Context(n, su.q, su.qm, su.rm, su.npin, precision=prec, e0=su.e0, b0=su.b0, **kw)
+        try:
+            ctx.init_device(spec, w.seed)
+            if eb is None:
+                eb = build_fields(ctx, su, n)
+            ctx.set_fields(eb)
+            ws = []
+            for _ in range(steps):
+                ctx.bfield()
+                ctx.grdcrt()
+                ctx.push_em(2)
+                ctx.bfield()
+                ctx.efield(tcs, 2.0)
+                fe = ctx.field_energy(su.cs)
+                ws.append(float(ctx.energy().sum() + fe[0] + fe[1]))
+        finally:
+            ctx.close()
+        ws = np.array(ws)
+        hist[name] = ws
+        out[name] = {"energy_first": ws[0], "energy_last": ws[-1],
+                     "max_rel_drift": float(np.abs(ws - ws[0]).max() / abs(ws[0]))}
+    out["max_rel_diff_fp32_fp64"] = float(np.abs(hist["fp32"] - hist["fp64"]).max()
+                                          / abs(hist["fp64"][0]))
+    return out
+
+
 def cpu_sample(w, su, nsample):
     """A bounded sample of the workload for the oracle: nsample particles split
     over the species, each species ordered by the reference's own
@@ -502,6 +550,8 @@ def main():
     ap.add_argument("--no-slab", action="store_true")
     ap.add_argument("--no-drift", action="store_true")
     ap.add_argument("--drift-steps", type=int, default=100)
+    ap.add_argument("--em-drift-steps", type=int, default=100,
+                    help="EM two-stream energy history (0: skip)")
     ap.add_argument("--slab-steps", type=int, default=40)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -612,6 +662,13 @@ def main():
         log(f"[bench] em: fp32 {em['fp32']['ms_per_step']:.3f} ms/step, kernel "
             f"{em['fp32']['kernel_ms']:.3f} ms ({em['fp32']['gbs_algorithmic']:.0f} GB/s)")
 
+    em_drift = None
+    if not args.no_em and not args.no_drift and args.em_drift_steps > 0:
+        em_drift = em_energy_drift(args.em_drift_steps)
+        log(f"[bench] em energy drift over {args.em_drift_steps} steps: fp32 "
+            f"{em_drift['fp32']['max_rel_drift']:.2e}, fp64 {em_drift['fp64']['max_rel_drift']:.2e}, "
+            f"forms differ by {em_drift['max_rel_diff_fp32_fp64']:.1e}")
+
     slab = None
     if not args.no_slab:
         slab = slab_leg(args.slab_steps, max(args.slab_steps // 8, 5) if not args.no_fp64 else 0,
@@ -658,6 +715,7 @@ def main():
                    "adjoint": adj,
                    "cpu_all_cores": cpu_all,
                    "energy_drift": drift,
+                   "em_energy_drift": em_drift,
                    "em_twostream": em,
                    "ramp_densest_slab": slab},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
