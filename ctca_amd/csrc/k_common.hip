@@ -230,14 +230,30 @@ __device__ __forceinline__ int64_t slab_key(const State64 &s, int64_t m, const G
 }
 
 // count the particles of each key; the count a particle finds is its rank
-// among the key's particles (kept for the scatter)
+// among the key's particles (kept for the scatter).  The input is the last
+// sort's order moved by a few pushes, so neighbouring lanes mostly hold the
+// same key: each run of equal keys in a wave takes its counts with one atomic
+// (its first lane adds the run's length) and ranks its lanes from the old
+// count, instead of one contended atomic per particle.
 __global__ __launch_bounds__(kBlock) void k_sort_hist(State64 s, SpeciesTab sp, Grid g, int64_t nkey,
                                                       int32_t *count, int32_t *rank) {
-  int is = blockIdx.y;
-  int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (m >= sp.cnt[is]) return;
-  m += sp.off[is];
-  rank[m] = atomicAdd(&count[(int64_t)is * nkey + slab_key(s, m, g)], 1);
+  const int is = blockIdx.y;
+  const int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const bool act = m < sp.cnt[is];  // the whole wave takes part in the shuffles
+  const int lane = __lane_id();
+  const int64_t key = act ? (int64_t)is * nkey + slab_key(s, m + sp.off[is], g) : -1 - lane;
+  const int64_t prev = __shfl_up(key, 1);
+  const bool head = lane == 0 || prev != key;
+  const unsigned long long heads = __ballot(head);
+  // the run's first lane: the highest head at or below this lane
+  const unsigned long long below = heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+  const int first = 63 - __clzll((long long)below);
+  const unsigned long long above = lane == 63 ? 0ull : heads & (~0ull << (lane + 1));
+  const int end = above ? __ffsll((long long)above) - 1 : 64;
+  int base = 0;
+  if (act && head) base = atomicAdd(&count[key], end - lane);
+  base = __shfl(base, first);
+  if (act) rank[m + sp.off[is]] = base + (lane - first);
 }
 
 // The destinations of a sort are a random permutation: written straight into
