@@ -20,7 +20,8 @@ ARCH = "gfx950"
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-munsafe-fp-atomics",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-          "-I", INC, "-I", CSRC, "-Wall", "-Wno-unused-function"]
+          "-I", INC, "-I", CSRC, "-Wall", "-Wno-unused-function",
+          "-Werror=misleading-indentation"]
 
 SOURCES = [
     ("k_fp64.hip", ["-ffp-contract=off"]),

@@ -495,6 +495,7 @@ int emg_upload_particles(emg_ctx *c, const emg_particle *pbuf, const int64_t *co
   }
   c->have_particles = true;
   c->pushes_since_sort = c->cfg.sort_every;  // unsorted: sort before the first push
+  c->moves_after_sort = emg::kUnsorted;
   c->rho_valid = false;
   c->j_valid = false;
   c->ke_valid = false;
@@ -734,10 +735,12 @@ int emg_positn(emg_ctx *c, int32_t ustep) {
   if (rc) return rc;
   if (ustep < 0) return set_error(EMG_ERR_ARG, "ustep must be >= 0");
   if ((rc = sort_if_due(c))) return rc;
-  if (c->cfg.precision == EMG_PLAIN_FP64)
+  if (c->cfg.precision == EMG_PLAIN_FP64) {
     launch_positn64(c, ustep);
-  else
+    if (c->moves_after_sort < emg::kUnsorted) c->moves_after_sort++;
+  } else {
     launch_fx_step(c, ustep, false, false, false);
+  }
   c->rho_valid = false;
   c->j_valid = false;
   return hip_check(hipGetLastError(), "positn launch");
@@ -752,6 +755,7 @@ int emg_push_es(emg_ctx *c, int32_t ustep) {
   if ((rc = sort_if_due(c))) return rc;
   if (c->cfg.precision == EMG_PLAIN_FP64) {
     launch_push_es64(c, ustep);
+    if (c->moves_after_sort < emg::kUnsorted) c->moves_after_sort++;
     c->rho_valid = false;
     c->ke_valid = false;
   } else {
@@ -846,6 +850,7 @@ int emg_push_em(emg_ctx *c, int32_t ustep) {
   if ((rc = sort_if_due(c))) return rc;
   if (c->cfg.precision == EMG_PLAIN_FP64) {
     launch_push_em64(c, ustep);
+    if (c->moves_after_sort < emg::kUnsorted) c->moves_after_sort++;
     c->ke_valid = false;
   } else {
     launch_fx_step(c, ustep, true, false, true);
@@ -924,6 +929,7 @@ int emg_state_load(emg_ctx *c, int32_t slot) {
     return set_error(EMG_ERR_ARG, "slot holds no saved state");
   for (int s = 0; s < c->sp.nspec; s++) c->sp.cnt[s] = c->slot_cnt[slot][s];
   c->have_particles = true;
+  c->moves_after_sort = emg::kUnsorted;
   c->rho_valid = c->j_valid = c->ke_valid = false;
   return copy_state(c->slot[slot], c->s64, c->cap_total, c->stream);
 }
@@ -1027,6 +1033,7 @@ int emg_init_device(emg_ctx *c, const emg_device_species *spec, int32_t nspec, u
   }
   c->have_particles = true;
   c->pushes_since_sort = c->cfg.sort_every;  // unsorted: sort before the first push
+  c->moves_after_sort = emg::kUnsorted;
   c->rho_valid = false;
   c->j_valid = false;
   c->ke_valid = false;

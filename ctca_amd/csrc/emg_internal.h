@@ -18,7 +18,8 @@ constexpr int kJShift = 24;  // current fixed point: 2^24 per unit flux (cell / 
 constexpr double kJScale = 16777216.0;
 constexpr int kCellBits = 10;  // packed cell index: ix | iy << 10 | iz << 20
 constexpr int kCellMask = (1 << kCellBits) - 1;
-constexpr int kStateSlots = 64;  // saved particle states of the adjoint window
+constexpr int kStateSlots = 64;
+constexpr int kUnsorted = 1 << 30;  // moves_after_sort of a state in no known order  // saved particle states of the adjoint window
 
 // device error flags
 constexpr int kFlagDomain = 1;    // particle outside [0, L] / non-finite
@@ -114,7 +115,8 @@ struct emg_ctx {
   emg::State64 s64;
   emg::State64 t64;  // sort scratch (lazy)
   bool have_particles;
-  int pushes_since_sort;  // PLAIN_FP64 pushes since the last cell sort
+  int pushes_since_sort;  // PLAIN_FP64 pushes since the last cell sort (sort_every)
+  int moves_after_sort;   // PLAIN_FP64 pushes since the state was last sorted, or kUnsorted
 
   // FP32 state
   emg::FxBuckets fx;

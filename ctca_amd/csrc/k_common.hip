@@ -281,6 +281,28 @@ __global__ __launch_bounds__(kBlock) void k_sort_scatter(State64 s, uint4 *rec, 
   r[3] = make_uint4((unsigned)s.pid[m], 0u, 0u, 0u);
 }
 
+// A re-sort a few pushes after the last one is nearly the identity: the lanes
+// of a wave land on neighbouring slots, so the particles go straight into the
+// seven arrays, coalesced, with no record pass.
+__global__ __launch_bounds__(kBlock) void k_sort_scatter_direct(State64 s, State64 t, SpeciesTab sp,
+                                                                Grid g, int64_t nkey,
+                                                                const int64_t *start,
+                                                                const int32_t *rank) {
+  int is = blockIdx.y;
+  int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (m >= sp.cnt[is]) return;
+  m += sp.off[is];
+  int64_t key = (int64_t)is * nkey + slab_key(s, m, g);
+  int64_t d = sp.off[is] + (start[key] - start[(int64_t)is * nkey]) + rank[m];
+  t.x[d] = s.x[m];
+  t.y[d] = s.y[m];
+  t.z[d] = s.z[m];
+  t.vx[d] = s.vx[m];
+  t.vy[d] = s.vy[m];
+  t.vz[d] = s.vz[m];
+  t.pid[d] = s.pid[m];
+}
+
 __global__ __launch_bounds__(kBlock) void k_sort_split(const uint4 *rec, State64 t, SpeciesTab sp) {
   int is = blockIdx.y;
   int64_t d = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -439,6 +461,9 @@ void launch_read_current(emg_ctx *c, int species, double *dst, const int lo[3], 
                        (const void *)c->jfx, dst);
 }
 
+// pushes after a sort up to which a re-sort writes the arrays directly
+constexpr int kResortDirect = 32;
+
 int sort64(emg_ctx *c) {
   int64_t nkey = c->ncell;
   int64_t len = nkey * c->sp.nspec;
@@ -482,11 +507,17 @@ int sort64(emg_ctx *c) {
   hipLaunchKernelGGL(k_sort_hist, grid, dim3(kBlock), 0, c->stream, c->s64, c->sp, c->g, nkey,
                      c->sort_count, rank);
   exclusive_scan_i32(c->sort_count, c->sort_start, len, c->scan_tmp, c->stream);
-  hipLaunchKernelGGL(k_sort_scatter, grid, dim3(kBlock), 0, c->stream, c->s64,
-                     (uint4 *)c->sort_rec, c->sp, c->g, nkey, c->sort_start, rank);
-  hipLaunchKernelGGL(k_sort_split, grid, dim3(kBlock), 0, c->stream, (const uint4 *)c->sort_rec,
-                     c->t64, c->sp);
+  if (c->moves_after_sort <= kResortDirect) {
+    hipLaunchKernelGGL(k_sort_scatter_direct, grid, dim3(kBlock), 0, c->stream, c->s64, c->t64,
+                       c->sp, c->g, nkey, c->sort_start, rank);
+  } else {
+    hipLaunchKernelGGL(k_sort_scatter, grid, dim3(kBlock), 0, c->stream, c->s64,
+                       (uint4 *)c->sort_rec, c->sp, c->g, nkey, c->sort_start, rank);
+    hipLaunchKernelGGL(k_sort_split, grid, dim3(kBlock), 0, c->stream,
+                       (const uint4 *)c->sort_rec, c->t64, c->sp);
+  }
   timer_end(c, kTSort);
+  c->moves_after_sort = 0;
   State64 tmp = c->s64;
   c->s64 = c->t64;
   c->t64 = tmp;

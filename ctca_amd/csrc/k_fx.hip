@@ -1530,6 +1530,8 @@ int fx_to64(emg_ctx *c, State64 &dst) {
 }
 
 void launch_fx_step(emg_ctx *c, int ustep, bool push, bool deposit, bool em) {
+  // the bucket arrays exist only in the FP32 form: launch nothing otherwise
+  if (c->cfg.precision != EMG_FP32_FIXED || c->fx.nb <= 0) return;
   FxBuckets &B = c->fx;
   const int dep = em ? 2 : (deposit ? 1 : 0);
   int in = B.cur, out = 1 - B.cur;
