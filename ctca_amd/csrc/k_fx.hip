@@ -654,17 +654,27 @@ __device__ __forceinline__ void block_reduce3(double ke[3], double *out) {
 }
 
 // XCD-aware bucket order.  Consecutive workgroups go round-robin to the 8
-// XCDs, so XCD x takes the contiguous range [x nb/8, (x+1) nb/8) of a work
-// order u that enumerates tiles y-slowest, then z, then x, with the species
-// of one tile adjacent.  Each XCD then holds a y-slab of tiles that spans
-// every z and every species: neighbouring tiles (and the species of one
-// tile) share field and charge halo nodes in that XCD's L2, and a density
-// that varies along z, or species of very different sizes, load the XCDs
-// evenly (a species-major or z-major split would leave some XCDs idle).
+// XCDs, so XCD x takes the workgroups bid = x (mod 8), in order.  Tiles are
+// enumerated y-slowest, then z, then x, and XCD x owns the eighth
+// [x, x + 1) * ntiles / 8 of them: a y-slab spanning every z, so a density
+// that varies along z loads the XCDs evenly and neighbouring tiles share halo
+// nodes in that XCD's L2.  Inside its slab an XCD runs all tiles of species
+// 0, then of species 1, and so on: the blocks that run side by side on a CU
+// then carry similar work.  (With the species of one tile adjacent, the EM
+// two-stream's 1.6K-particle beam blocks ran beside 16.4K-particle background
+// blocks, and the CU slots they held did little: 6.86 ms against 4.67 ms.)
 __device__ __forceinline__ int bucket_of_block(int bid, int nb, int nspec, int ntx, int ntz,
                                                int ntiles) {
-  const int u = (nb & 7) ? bid : (bid & 7) * (nb >> 3) + (bid >> 3);
-  const int s = u % nspec, v = u / nspec;
+  (void)nb;
+  int s, v;
+  if ((ntiles & 7) == 0) {
+    const int per = ntiles >> 3, x = bid & 7, k = bid >> 3;
+    s = k / per;
+    v = x * per + (k - s * per);
+  } else {
+    s = bid % nspec;
+    v = bid / nspec;
+  }
   const int tx = v % ntx, r = v / ntx;
   const int tz = r % ntz, ty = r / ntz;
   return s * ntiles + (tz * (ntiles / (ntx * ntz)) + ty) * ntx + tx;
