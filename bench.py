@@ -29,6 +29,8 @@ Further legs, reported inside `config` (each can be switched off):
     update on the device (bfield, grdcrt, push, bfield, efield);
   * ramp_densest_slab: the densest 512x512x32 z-slab of the 1G exp(z) ramp
     (474,679,216 particles), the worst-loaded share of 8 GPUs;
+  * ramp_full_1G: the whole 512x512x256 exp(z) ramp, 1,073,741,824 particles
+    on one GPU, both forms (one context after the other);
   * energy_drift: the plain ES configuration (32^3, 1,048,576 particles) run
     self-consistently on the device (deposit, field solve, push) in both
     forms, with the drift of kinetic + field energy;
@@ -261,12 +263,15 @@ def em_leg(steps32, steps64, warmup, tile_arg=0):
     return out
 
 
-def slab_leg(steps32, steps64, warmup, tile_arg=0):
-    """The densest z-slab of the 1G ramp (512x512x32 cells, 474,679,216
-    particles: the share of one of 8 GPUs that carries the most particles):
-    ES push + fixed-point charge in the FP32 form, the FP64 form beside it."""
+def slab_leg(steps32, steps64, warmup, tile_arg=0, workload=None):
+    """An ES workload timed like the headline: by default the densest z-slab
+    of the 1G ramp (512x512x32 cells, 474,679,216 particles: the share of one
+    of 8 GPUs that carries the most particles); with workload=units.ramp512
+    the whole ramp (512x512x256, 1,073,741,824 particles) on one GPU.  ES
+    push + fixed-point charge in the FP32 form, the FP64 form beside it (the
+    two contexts one after the other)."""
     from ctca_amd import emg, units
-    w = units.ramp512_slab()
+    w = (workload or units.ramp512_slab)()
     su = units.to_sunits(w)
     n = w.n
     nsp = len(w.species)
@@ -553,6 +558,9 @@ def main():
     ap.add_argument("--em-drift-steps", type=int, default=100,
                     help="EM two-stream energy history (0: skip)")
     ap.add_argument("--slab-steps", type=int, default=40)
+    ap.add_argument("--no-full-ramp", action="store_true",
+                    help="skip the whole 512x512x256 ramp (1,073,741,824 particles)")
+    ap.add_argument("--full-ramp-steps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the particle path runs on one MI355X that owns the "
@@ -676,6 +684,14 @@ def main():
         log(f"[bench] ramp slab: fp32 {slab['fp32']['ms_per_step']:.3f} ms/step, kernel "
             f"{slab['fp32']['kernel_ms']:.3f} ms ({slab['fp32']['gbs_algorithmic']:.0f} GB/s)")
 
+    full_ramp = None
+    if not args.no_full_ramp:
+        full_ramp = slab_leg(args.full_ramp_steps,
+                             max(args.full_ramp_steps // 4, 3) if not args.no_fp64 else 0,
+                             args.warmup, args.tile, workload=units.ramp512)
+        log(f"[bench] full ramp: fp32 {full_ramp['fp32']['ms_per_step']:.3f} ms/step, kernel "
+            f"{full_ramp['fp32']['kernel_ms']:.3f} ms ({full_ramp['fp32']['gbs_algorithmic']:.0f} GB/s)")
+
     cpu = cpu_all = None
     if not args.no_cpu:
         cpu = cpu_baseline(w, su, eb, args.cpu_sample, args.cpu_steps)
@@ -717,7 +733,8 @@ def main():
                    "energy_drift": drift,
                    "em_energy_drift": em_drift,
                    "em_twostream": em,
-                   "ramp_densest_slab": slab},
+                   "ramp_densest_slab": slab,
+                   "ramp_full_1G": full_ramp},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "copy_gbs": copy_gbs, "frac_of_copy": achieved / copy_gbs,

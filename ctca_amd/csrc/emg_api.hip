@@ -279,7 +279,8 @@ int emg_create(const emg_config *cfg, emg_ctx **out) {
   c->sp.off[cfg->nspec] = off;
   c->cap_total = off;
   int nsp = cfg->nspec;
-  if ((rc = dalloc(&c->dflag, 1, "flag")) || (rc = dalloc(&c->ke_out, 3 * nsp, "energy"))) {
+  if ((rc = dalloc(&c->dflag, 1, "flag")) || (rc = dalloc(&c->ke_out, 3 * nsp, "energy")) ||
+      (rc = dalloc((double **)&c->face, (size_t)(face_len(c->g) * nsp), "upper-face charge"))) {
     emg_destroy(c);
     return rc;
   }
@@ -365,6 +366,7 @@ int emg_destroy(emg_ctx *c) {
   dfree(c->jfx);
   dfree(c->ke_part);
   dfree(c->ke_out);
+  dfree(c->face);
   dfree(c->sort_count);
   dfree(c->sort_cursor);
   dfree(c->sort_start);
@@ -497,6 +499,7 @@ int emg_upload_particles(emg_ctx *c, const emg_particle *pbuf, const int64_t *co
   c->pushes_since_sort = c->cfg.sort_every;  // unsorted: sort before the first push
   c->moves_after_sort = emg::kUnsorted;
   c->rho_valid = false;
+  c->faces_valid = false;
   c->j_valid = false;
   c->ke_valid = false;
   return EMG_OK;
@@ -742,6 +745,7 @@ int emg_positn(emg_ctx *c, int32_t ustep) {
     launch_fx_step(c, ustep, false, false, false);
   }
   c->rho_valid = false;
+  c->faces_valid = false;
   c->j_valid = false;
   return hip_check(hipGetLastError(), "positn launch");
 }
@@ -757,10 +761,12 @@ int emg_push_es(emg_ctx *c, int32_t ustep) {
     launch_push_es64(c, ustep);
     if (c->moves_after_sort < emg::kUnsorted) c->moves_after_sort++;
     c->rho_valid = false;
+    c->faces_valid = false;
     c->ke_valid = false;
   } else {
     launch_fx_step(c, ustep, true, true, false);
     c->rho_valid = true;
+    c->faces_valid = false;
     c->ke_valid = true;  // fused partials of the new velocities
   }
   c->j_valid = false;
@@ -778,11 +784,15 @@ int emg_deposit_rho(emg_ctx *c, int32_t func) {
     launch_fx_deposit(c);
   }
   c->rho_valid = true;
+  c->faces_valid = false;
   return hip_check(hipGetLastError(), "deposit launch");
 }
 
+// Read a grid array back: the read-out kernel writes it into a device staging
+// buffer (allocated for the call, or `stage` of `stage_cap` doubles), which
+// is copied to the host.
 static int read_grid(emg_ctx *c, double *host, const int32_t lo[3], const int32_t hi[3], int comps,
-                     int species, bool current) {
+                     int species, bool current, double *stage = nullptr, int64_t stage_cap = 0) {
   if (!host || !lo || !hi) return set_error(EMG_ERR_ARG, "null argument");
   int l[3], h[3];
   int64_t n = comps;
@@ -792,16 +802,25 @@ static int read_grid(emg_ctx *c, double *host, const int32_t lo[3], const int32_
     h[d] = hi[d];
     n *= (int64_t)(hi[d] - lo[d] + 1);
   }
-  double *d_out = nullptr;
-  int rc = dalloc(&d_out, (size_t)n, "read-out");
-  if (rc) return rc;
+  double *d_out = stage;
+  int rc;
+  if (stage) {
+    if (n > stage_cap)
+      return set_error(EMG_ERR_CAPACITY, "the read-out exceeds the library's static staging "
+                                         "array (2^24 values, 128 MiB): read it with emg_get_rho");
+  } else if ((rc = dalloc(&d_out, (size_t)n, "read-out"))) {
+    return rc;
+  }
   if (current)
     launch_read_current(c, species, d_out, l, h);
   else
     launch_read_rho(c, species, d_out, l, h);
   rc = check_flags(c);
-  if (!rc) rc = hip_check(hipMemcpy(host, d_out, sizeof(double) * n, hipMemcpyDeviceToHost), "read-out copy");
-  dfree(d_out);
+  if (!rc)
+    rc = hip_check(hipMemcpyAsync(host, d_out, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream),
+                   "read-out copy");
+  if (!rc) rc = hip_check(hipStreamSynchronize(c->stream), "read-out copy");
+  if (!stage) dfree(d_out);
   return rc;
 }
 
@@ -858,6 +877,7 @@ int emg_push_em(emg_ctx *c, int32_t ustep) {
   }
   c->j_valid = true;
   c->rho_valid = false;
+  c->faces_valid = false;
   return hip_check(hipGetLastError(), "push_em launch");
 }
 
@@ -931,6 +951,7 @@ int emg_state_load(emg_ctx *c, int32_t slot) {
   c->have_particles = true;
   c->moves_after_sort = emg::kUnsorted;
   c->rho_valid = c->j_valid = c->ke_valid = false;
+  c->faces_valid = false;
   return copy_state(c->slot[slot], c->s64, c->cap_total, c->stream);
 }
 
@@ -1035,6 +1056,7 @@ int emg_init_device(emg_ctx *c, const emg_device_species *spec, int32_t nspec, u
   c->pushes_since_sort = c->cfg.sort_every;  // unsorted: sort before the first push
   c->moves_after_sort = emg::kUnsorted;
   c->rho_valid = false;
+  c->faces_valid = false;
   c->j_valid = false;
   c->ke_valid = false;
   return EMG_OK;

@@ -99,6 +99,29 @@ struct Timer {
 
 struct EsSolver;  // device field solve (k_poisson.hip)
 
+// Upper-face accumulators of the charge.  The periodic accumulators add the
+// weight a particle puts on node n of an axis into node 0; the reference
+// keeps it apart until its border exchange and add_boundary_charge, which
+// leave the raw deposit of node n in ghost plane -1 and that of node 0 in
+// plane n + 1.  A face array per species holds the raw weights of the nodes
+// with index n on some axis (filled on demand for a read-out of the ghost
+// planes): face X = nodes (nx, b, c), b in 0..ny, c in 0..nz; face Y = nodes
+// (a, ny, c), a in 0..nx-1, c in 0..nz; face Z = nodes (a, b, nz), a in
+// 0..nx-1, b in 0..ny-1.
+__host__ __device__ inline int64_t face_len(const Grid &g) {
+  return (int64_t)(g.ny + 1) * (g.nz + 1) + (int64_t)g.nx * (g.nz + 1) + (int64_t)g.nx * g.ny;
+}
+
+// face slot of node (a, b, c), a in 0..nx, b in 0..ny, c in 0..nz; -1 when no
+// index is n
+__host__ __device__ inline int64_t face_index(const Grid &g, int a, int b, int c) {
+  if (a == g.nx) return (int64_t)c * (g.ny + 1) + b;
+  const int64_t y0 = (int64_t)(g.ny + 1) * (g.nz + 1);
+  if (b == g.ny) return y0 + (int64_t)c * g.nx + a;
+  if (c == g.nz) return y0 + (int64_t)g.nx * (g.nz + 1) + (int64_t)b * g.nx + a;
+  return -1;
+}
+
 }  // namespace emg
 
 struct emg_ctx {
@@ -140,6 +163,8 @@ struct emg_ctx {
   unsigned long long *jfx;     // nspec x 3 x ncell fixed point
   bool rho_valid;
   bool j_valid;
+  void *face;       // nspec x face_len: upper-face charge (double, or fixed point in FP32_FIXED)
+  bool faces_valid; // face holds the deposit of the current positions
 
   // energy
   double *ke_part;  // partial sums
@@ -195,6 +220,7 @@ void launch_positn64(emg_ctx *c, int ustep);
 void launch_push_es64(emg_ctx *c, int ustep);
 void launch_deposit64(emg_ctx *c);
 void launch_push_em64(emg_ctx *c, int ustep);
+void launch_faces64(emg_ctx *c);  // c->face (zeroed) += upper-face charge
 // device Yee update and field energy on eb_dev (k_yee.hip)
 void launch_bfield(emg_ctx *c);
 void launch_efield(emg_ctx *c, double tcs, double dlt);
@@ -228,6 +254,7 @@ int fx_to64(emg_ctx *c, State64 &dst);               // convert back (species co
 void launch_fx_step(emg_ctx *c, int ustep, bool push, bool deposit, bool em);
 void launch_fx_deposit(emg_ctx *c);
 void launch_fx_energy(emg_ctx *c);
+void launch_fx_faces(emg_ctx *c);  // c->face (zeroed) += upper-face charge, fixed point
 void fx_energy_from_push(emg_ctx *c);
 int fx_rebucket(emg_ctx *c);
 

@@ -154,15 +154,85 @@ struct ReadArgs {
   int64_t ncell;
 };
 
-__global__ __launch_bounds__(kBlock) void k_read_rho64(ReadArgs r, const double *rho, double *out) {
+// The raw (unfolded) deposit of species accumulator P at node (a, b, c), a in
+// 0..nx, b in 0..ny, c in 0..nz: nodes with an index n are in the face array
+// F; any other node is its periodic accumulator less the face nodes folded
+// into it (its images with a 0 index replaced by n).
+template <class V>
+__device__ __forceinline__ V raw_node(const V *P, const V *F, const Grid &g, int a, int b, int c) {
+  const int64_t fi = face_index(g, a, b, c);
+  if (fi >= 0) return F[fi];
+  V v = P[((int64_t)c * g.ny + b) * g.nx + a];
+#pragma unroll
+  for (int m = 1; m < 8; m++) {
+    if (((m & 1) && a != 0) || ((m & 2) && b != 0) || ((m & 4) && c != 0)) continue;
+    v = v - F[face_index(g, (m & 1) ? g.nx : a, (m & 2) ? g.ny : b, (m & 4) ? g.nz : c)];
+  }
+  return v;
+}
+
+// The raw nodes whose sum the reference leaves at index t of an axis after
+// oh3_exchange_borders(CRH) + add_boundary_charge (ohhelp3.c:1010-1031,
+// charge.F90:265-409): the exchange copies node n into ghost -1 and node 0
+// into ghost n + 1; the fold then adds ghost -1 into node 0 and ghost n + 1
+// into node n, on the z axis over every (x, y) plane index, on y only for z
+// in 0..n, on x only for y and z in 0..n.
+__device__ __forceinline__ int fold_set(int t, int n, bool fold, int set[2]) {
+  set[1] = 0;
+  if (t == -1) { set[0] = n; return 1; }
+  if (t == n + 1) { set[0] = 0; return 1; }
+  if (fold && (t == 0 || t == n)) { set[0] = 0; set[1] = n; return 2; }
+  set[0] = t;
+  return 1;
+}
+
+// the reference's post-fold value at (i, j, k), each in -1..n+1, of species
+// accumulator P with faces F
+template <class V>
+__device__ __forceinline__ V ghost_value(const V *P, const V *F, const Grid &g, int i, int j,
+                                         int k) {
+  const bool kin = k >= 0 && k <= g.nz, jin = j >= 0 && j <= g.ny, iin = i >= 0 && i <= g.nx;
+  int sz[2], sy[2], sx[2];
+  const int nz = fold_set(k, g.nz, kin, sz);
+  const int ny = fold_set(j, g.ny, jin && kin, sy);
+  const int nx = fold_set(i, g.nx, iin && jin && kin, sx);
+  V v = 0;
+  for (int c = 0; c < nz; c++)
+    for (int b = 0; b < ny; b++)
+      for (int a = 0; a < nx; a++) v = v + raw_node(P, F, g, sx[a], sy[b], sz[c]);
+  return v;
+}
+
+// rho(1, lo:hi)^3.  Nodes 0..n: the periodic fold (node n = node 0).  Ghost
+// planes -1 and n + 1 (with the faces filled): what the reference's border
+// exchange and add_boundary_charge leave there.  Indices beyond -1..n+1, or
+// ghosts without faces: periodic images.
+__device__ __forceinline__ bool ghost_node(const ReadArgs &r, int i, int j, int k) {
+  const bool in = i >= 0 && i <= r.g.nx && j >= 0 && j <= r.g.ny && k >= 0 && k <= r.g.nz;
+  const bool ext = i >= -1 && i <= r.g.nx + 1 && j >= -1 && j <= r.g.ny + 1 && k >= -1 &&
+                   k <= r.g.nz + 1;
+  return !in && ext;
+}
+
+__global__ __launch_bounds__(kBlock) void k_read_rho64(ReadArgs r, const double *rho,
+                                                      const double *face, int64_t nf,
+                                                      double *out) {
   int64_t n = (int64_t)r.ext[0] * r.ext[1] * r.ext[2];
   int64_t o = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (o >= n) return;
   int i = (int)(o % r.ext[0]) + r.lo[0];
   int j = (int)((o / r.ext[0]) % r.ext[1]) + r.lo[1];
   int k = (int)(o / ((int64_t)r.ext[0] * r.ext[1])) + r.lo[2];
-  int64_t c = ((int64_t)pmod(k, r.g.nz) * r.g.ny + pmod(j, r.g.ny)) * r.g.nx + pmod(i, r.g.nx);
   double v = 0.0;
+  if (face && ghost_node(r, i, j, k)) {
+    for (int s = 0; s < r.nspec; s++) {
+      if (r.species >= 0 && s != r.species) continue;
+      v = v + ghost_value(rho + (int64_t)s * r.ncell, face + (int64_t)s * nf, r.g, i, j, k);
+    }
+    out[o] = v;
+    return;
+  }
+  int64_t c = ((int64_t)pmod(k, r.g.nz) * r.g.ny + pmod(j, r.g.ny)) * r.g.nx + pmod(i, r.g.nx);
   if (r.species >= 0) {
     v = rho[(int64_t)r.species * r.ncell + c];
   } else {
@@ -172,6 +242,7 @@ __global__ __launch_bounds__(kBlock) void k_read_rho64(ReadArgs r, const double 
 }
 
 __global__ __launch_bounds__(kBlock) void k_read_rhofx(ReadArgs r, const unsigned long long *rho,
+                                                      const unsigned long long *face, int64_t nf,
                                                       double *out) {
   int64_t n = (int64_t)r.ext[0] * r.ext[1] * r.ext[2];
   int64_t o = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -179,11 +250,14 @@ __global__ __launch_bounds__(kBlock) void k_read_rhofx(ReadArgs r, const unsigne
   int i = (int)(o % r.ext[0]) + r.lo[0];
   int j = (int)((o / r.ext[0]) % r.ext[1]) + r.lo[1];
   int k = (int)(o / ((int64_t)r.ext[0] * r.ext[1])) + r.lo[2];
+  const bool ghost = face && ghost_node(r, i, j, k);
   int64_t c = ((int64_t)pmod(k, r.g.nz) * r.g.ny + pmod(j, r.g.ny)) * r.g.nx + pmod(i, r.g.nx);
   double v = 0.0;
   for (int s = 0; s < r.nspec; s++) {
     if (r.species >= 0 && s != r.species) continue;
-    long long w = (long long)rho[(int64_t)s * r.ncell + c];
+    long long w = ghost ? (long long)ghost_value(rho + (int64_t)s * r.ncell, face + (int64_t)s * nf,
+                                                 r.g, i, j, k)
+                        : (long long)rho[(int64_t)s * r.ncell + c];
     v = v + r.q[s] * ((double)w * (1.0 / kFixScale));
   }
   out[o] = v;
@@ -443,10 +517,27 @@ void launch_read_rho(emg_ctx *c, int species, double *dst, const int lo[3], cons
   ReadArgs r = read_args(c, species, lo, hi);
   int64_t n = (int64_t)r.ext[0] * r.ext[1] * r.ext[2];
   unsigned grid = (unsigned)((n + kBlock - 1) / kBlock);
+  // the reference's ghost planes need the upper-face deposit of the current
+  // positions: computed once per deposit, when a read-out first asks for them
+  bool ghosts = false;
+  for (int d = 0; d < 3; d++) {
+    const int nd = d == 0 ? c->g.nx : (d == 1 ? c->g.ny : c->g.nz);
+    ghosts = ghosts || (lo[d] <= -1 && hi[d] >= -1) || (lo[d] <= nd + 1 && hi[d] >= nd + 1);
+  }
+  if (ghosts && c->face && !c->faces_valid) {
+    if (c->cfg.precision == EMG_PLAIN_FP64)
+      launch_faces64(c);
+    else
+      launch_fx_faces(c);
+    c->faces_valid = true;
+  }
+  const int64_t nf = face_len(c->g);
   if (c->cfg.precision == EMG_PLAIN_FP64)
-    hipLaunchKernelGGL(k_read_rho64, dim3(grid), dim3(kBlock), 0, c->stream, r, c->rho64, dst);
+    hipLaunchKernelGGL(k_read_rho64, dim3(grid), dim3(kBlock), 0, c->stream, r, c->rho64,
+                       ghosts ? (const double *)c->face : nullptr, nf, dst);
   else
-    hipLaunchKernelGGL(k_read_rhofx, dim3(grid), dim3(kBlock), 0, c->stream, r, c->rhofx, dst);
+    hipLaunchKernelGGL(k_read_rhofx, dim3(grid), dim3(kBlock), 0, c->stream, r, c->rhofx,
+                       ghosts ? (const unsigned long long *)c->face : nullptr, nf, dst);
 }
 
 void launch_read_current(emg_ctx *c, int species, double *dst, const int lo[3], const int hi[3]) {

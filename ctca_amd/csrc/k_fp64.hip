@@ -569,6 +569,40 @@ dim3 species_grid(const SpeciesTab &sp) {
   return dim3((unsigned)((mx + kBlock - 1) / kBlock), (unsigned)sp.nspec, 1);
 }
 
+
+// The weights that the reference's charge puts on nodes with index n on some
+// axis (charge.F90:152-210 with int(): a particle in cell n - 1, or at a
+// position of exactly L, reaches them), kept per species in the upper-face
+// arrays.  A weight on node n + 1 (only a position of exactly L has one, of
+// size zero) is dropped, as the reference's border exchange overwrites that
+// node.
+__global__ __launch_bounds__(kBlock) void k_faces64(State64 s, SpeciesTab sp, Grid g,
+                                                    double *__restrict__ face, int64_t nf) {
+  const int is = blockIdx.y;
+  int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (m >= sp.cnt[is]) return;
+  m += sp.off[is];
+  double x = s.x[m], y = s.y[m], z = s.z[m];
+  x = x >= 0.0 ? (x <= g.slx ? x : g.slx) : 0.0;
+  y = y >= 0.0 ? (y <= g.sly ? y : g.sly) : 0.0;
+  z = z >= 0.0 ? (z <= g.slz ? z : g.slz) : 0.0;
+  const int i = (int)x, j = (int)y, k = (int)z;
+  if (i < g.nx - 1 && j < g.ny - 1 && k < g.nz - 1) return;
+  const Cic w = cic(x, y, z, i, j, k);
+  const double qc = sp.q[is];
+  double *f = face + (int64_t)is * nf;
+  const int ci[8] = {i, i + 1, i + 1, i, i, i + 1, i + 1, i};
+  const int cj[8] = {j, j, j + 1, j + 1, j, j, j + 1, j + 1};
+  const int ck[8] = {k + 1, k + 1, k + 1, k + 1, k, k, k, k};
+  const double v[8] = {w.v1, w.v2, w.v3, w.v4, w.v5, w.v6, w.v7, w.v8};
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    if (ci[q] > g.nx || cj[q] > g.ny || ck[q] > g.nz) continue;
+    const int64_t fi = face_index(g, ci[q], cj[q], ck[q]);
+    if (fi >= 0) atomicAdd(f + fi, v[q] * qc);
+  }
+}
+
 }  // namespace
 
 void launch_relocate64(emg_ctx *c, int elo[3], int eext[3]) {
@@ -619,6 +653,15 @@ void launch_push_em64(emg_ctx *c, int ustep) {
   hipLaunchKernelGGL(k_push_em64, grid, dim3(kBlock), 0, c->stream, c->s64, c->sp, c->node64,
                      c->g, ustep, c->j64, c->ncell, c->dflag);
   timer_end(c, kTPush);
+}
+
+void launch_faces64(emg_ctx *c) {
+  const int64_t nf = face_len(c->g);
+  launch_zero(c->face, sizeof(double) * nf * c->sp.nspec, c->stream);
+  int64_t mx = 1;
+  for (int q = 0; q < c->sp.nspec; q++) mx = c->sp.cnt[q] > mx ? c->sp.cnt[q] : mx;
+  hipLaunchKernelGGL(k_faces64, dim3((unsigned)((mx + kBlock - 1) / kBlock), c->sp.nspec),
+                     dim3(kBlock), 0, c->stream, c->s64, c->sp, c->g, (double *)c->face, nf);
 }
 
 }  // namespace emg

@@ -58,7 +58,7 @@ __host__ __device__ constexpr int64_t align_up(int64_t v) { return (v + kAlign -
 constexpr int kCodes = 27;    // direction codes (dz+1)*9 + (dy+1)*3 + (dx+1); 13 = same tile
 constexpr int kFar = 27;      // not a neighbouring tile
 constexpr int kPre = 28;      // prefix table entries per bucket
-constexpr uint32_t kNoStore = 0x80000000u;  // buffer offset past every array: store dropped
+constexpr uint32_t kNoStore = 0x80000000u;  // buffer offset past every bucket: store dropped
 
 __device__ __forceinline__ int wrapc(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
@@ -86,7 +86,6 @@ struct FxArgs {
   int32_t *pre_out;
   FxRec *ob_out;
   int64_t area, spill_cap;
-  int32_t nrec;  // bytes per particle array of a buffer (bound of the masked stores)
   const float *node32;
   unsigned long long *rho;   // charge, nspec x ncell
   unsigned long long *jcur;  // zigzag current, nspec x 3 x ncell
@@ -114,8 +113,10 @@ __device__ __forceinline__ void load_pcl(const StateFx &in, int64_t slot, Pcl &p
 }
 
 // The hot loop addresses every particle array with one 32-bit byte offset
-// (slot * 4, slots < 2^29) added to the array's uniform base, so each load
-// needs no per-array 64-bit address arithmetic.
+// (slot * 4, the slot counted from the bucket's first slot) added to the
+// array's base shifted to that bucket (uniform over the block), so each load
+// needs no per-array 64-bit address arithmetic and the context's slot count
+// is not bounded by the 32-bit offsets.
 template <class U>
 __device__ __forceinline__ U &at(U *base, uint32_t off) {
   return *reinterpret_cast<U *>(reinterpret_cast<char *>(base) + off);
@@ -133,6 +134,20 @@ __device__ __forceinline__ void load_off(const StateFx &in, uint32_t off, Pcl &p
   p.vz = at(in.vz, off);
 }
 
+// the state arrays shifted to slot `first` (a bucket's first slot)
+__device__ __forceinline__ StateFx shifted(const StateFx &s, int64_t first) {
+  StateFx r;
+  r.cell = s.cell + first;
+  r.fx = s.fx + first;
+  r.fy = s.fy + first;
+  r.fz = s.fz + first;
+  r.vx = s.vx + first;
+  r.vy = s.vy + first;
+  r.vz = s.vz + first;
+  r.pid = s.pid ? s.pid + first : nullptr;
+  return r;
+}
+
 __device__ __forceinline__ void load_rec(const FxRec *ob, int64_t k, Pcl &p) {
   const float4 *q = reinterpret_cast<const float4 *>(ob + k);
   float4 h0 = q[0], h1 = q[1];
@@ -146,9 +161,10 @@ __device__ __forceinline__ void load_rec(const FxRec *ob, int64_t k, Pcl &p) {
   p.pid = __float_as_int(h1.w);
 }
 
-// Masked store of one value per lane through a buffer resource: a lane whose
-// offset is kNoStore lies outside the buffer and its store is dropped by the
-// hardware range check, so the instruction count never depends on the data.
+// Masked store of one value per lane through a buffer resource over one
+// bucket's slots (nrec bytes): a lane whose offset is kNoStore lies outside
+// it and its store is dropped by the hardware range check, so the
+// instruction count never depends on the data.
 __device__ __forceinline__ void bstore(void *base, int32_t nrec, uint32_t off, uint32_t v) {
   __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(base, 0, nrec, 0x00020000);
   __builtin_amdgcn_raw_buffer_store_b32(v, r, (int)off, 0, 0);
@@ -475,10 +491,11 @@ __device__ __forceinline__ void zig_seg_global(unsigned long long *j3, int64_t n
 // DEP: 0 no deposit, 1 charge (psolve2 + charge), 2 zigzag current (psolve1).
 template <int T, bool PUSH, int DEP, bool PID>
 __device__ __forceinline__ void fx_one(const FxArgs &a, int s, bool act, const Pcl &p, int ox,
-                                       int oy, int oz, bool lds_ok, uint32_t stay_base,
+                                       int oy, int oz, bool lds_ok, const StateFx &outb,
+                                       int32_t nrec,
                                        int *n_stay, int *n_out, FxRec *outbox,
                                        unsigned char *ocode, const float4 *fA, const float2 *fB,
-                                       unsigned long long *rh, float ke[3]) {
+                                       unsigned long long *rh, double ke[3]) {
   constexpr int N1 = T + 1, N3 = T + 3;
   const Grid &g = a.g;
   int ix = p.cell & kCellMask, iy = (p.cell >> kCellBits) & kCellMask,
@@ -502,10 +519,13 @@ __device__ __forceinline__ void fx_one(const FxArgs &a, int s, bool act, const P
   }
   if (PUSH) {
     boris32(vx, vy, vz, f, a.qm[s] * a.gstep * 0.5f);
-    float ax = act ? vx : 0.f, ay = act ? vy : 0.f, az = act ? vz : 0.f;
-    ke[0] = fmaf(ax, ax, ke[0]);
-    ke[1] = fmaf(ay, ay, ke[1]);
-    ke[2] = fmaf(az, az, ke[2]);
+    // kinetic energy in FP64, as energy.F90:38-43 sums it: the square of an
+    // FP32 velocity is exact in double, so only the running sum rounds
+    const double ax = act ? (double)vx : 0.0, ay = act ? (double)vy : 0.0,
+                 az = act ? (double)vz : 0.0;
+    ke[0] = fma(ax, ax, ke[0]);
+    ke[1] = fma(ay, ay, ke[1]);
+    ke[2] = fma(az, az, ke[2]);
   }
   const float fbx = fx, fby = fy, fbz = fz;
   const int ibx = ix, iby = iy, ibz = iz;
@@ -592,7 +612,7 @@ __device__ __forceinline__ void fx_one(const FxArgs &a, int s, bool act, const P
     sbase = __builtin_amdgcn_readlane(sbase, sleader);
     int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
                                          __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
-    if (stay) off = (stay_base + (uint32_t)(sbase + rank)) << 2;
+    if (stay) off = (uint32_t)(sbase + rank) << 2;
   }
   const bool mover = act && !stay;
   uint64_t mv = __ballot(mover);
@@ -631,7 +651,7 @@ __device__ __forceinline__ void fx_one(const FxArgs &a, int s, bool act, const P
       }
     }
   }
-  store_masked<PID>(a.out, a.nrec, off, cell, p.pid, fx, fy, fz, vx, vy, vz);
+  store_masked<PID>(outb, nrec, off, cell, p.pid, fx, fy, fz, vx, vy, vz);
 }
 
 template <int NT>
@@ -738,7 +758,7 @@ __global__ __launch_bounds__(T == 8 ? 512 : 256) void k_fx_step(FxArgs a) {
   __shared__ int seg_pre[kPre];
   __shared__ int n_stay, n_out;
   __shared__ int hist[kPre], opre[kPre];
-  float ke[3] = {0.f, 0.f, 0.f};  // per lane: at most a few hundred terms
+  double ke[3] = {0.0, 0.0, 0.0};  // per lane, FP64
   const int bid = blockIdx.x;
   int part = bid;  // energy partial slot: bucket index, then spill blocks
   int s = 0, b = -1;
@@ -755,14 +775,16 @@ __global__ __launch_bounds__(T == 8 ? 512 : 256) void k_fx_step(FxArgs a) {
     const int ox = (t % a.ntx) * T, oy = ((t / a.ntx) % a.nty) * T, oz = (t / (a.ntx * a.nty)) * T;
     const int cnt = a.count_in[b];
     const int nst = a.nst_in[b];
-    const uint32_t beg32 = (uint32_t)a.start_in[b];
     const uint32_t gap32 = (uint32_t)(a.cap_in[b] - cnt);
-    const uint32_t stay_base = (uint32_t)a.start_out[b];
+    // input and output state seen from the bucket's first slot
+    const StateFx inb = shifted(a.in, a.start_in[b]);
+    const StateFx outb = shifted(a.out, a.start_out[b]);
+    const int32_t nrec = a.cap_out[b] * 4;
     auto off_of = [&](int i) -> uint32_t {
-      return (beg32 + (uint32_t)i + (i >= nst ? gap32 : 0u)) << 2;
+      return ((uint32_t)i + (i >= nst ? gap32 : 0u)) << 2;
     };
     auto ld = [&](int i, Pcl &p) {
-      load_off<PID>(a.in, off_of(min(i + (int)threadIdx.x, cnt - 1)), p);
+      load_off<PID>(inb, off_of(min(i + (int)threadIdx.x, cnt - 1)), p);
     };
     // particle j of the block's list: the own particles, the arrivals pulled
     // from the neighbours' outboxes (needs the pull table), then the previous
@@ -773,7 +795,7 @@ __global__ __launch_bounds__(T == 8 ? 512 : 256) void k_fx_step(FxArgs a) {
     auto ldx = [&](int i, Pcl &p) {
       const int j = i + (int)threadIdx.x;
       if (j < cnt) {
-        load_off<PID>(a.in, off_of(j), p);
+        load_off<PID>(inb, off_of(j), p);
       } else if (j < cnt + narr) {
         load_rec(a.ob_in, arrival_rec(seg_src, seg_pre, j - cnt), p);
         if (!PID) p.pid = 0;
@@ -783,7 +805,7 @@ __global__ __launch_bounds__(T == 8 ? 512 : 256) void k_fx_step(FxArgs a) {
       }
     };
     auto one = [&](bool act, const Pcl &p) {
-      fx_one<T, PUSH, DEP, PID>(a, s, act, p, ox, oy, oz, true, stay_base, &n_stay, &n_out,
+      fx_one<T, PUSH, DEP, PID>(a, s, act, p, ox, oy, oz, true, outb, nrec, &n_stay, &n_out,
                                     outbox, ocode, fA, fB, rh, ke);
     };
     // Software pipelined three deep: while a lane pushes particle k, the loads
@@ -910,8 +932,7 @@ __global__ __launch_bounds__(T == 8 ? 512 : 256) void k_fx_step(FxArgs a) {
     }
   }
   if (PUSH) {
-    double kd[3] = {(double)ke[0], (double)ke[1], (double)ke[2]};
-    block_reduce3<NT>(kd, a.ke_part + 3 * (int64_t)part);
+    block_reduce3<NT>(ke, a.ke_part + 3 * (int64_t)part);
   }
 }
 
@@ -993,6 +1014,64 @@ __global__ __launch_bounds__(kBlock) void k_fx_deposit(FxArgs a) {
       int W[8];
       fixed_weights(p.fx, p.fy, p.fz, W);
       deposit_global(r, ix, iy, iz, W);
+    }
+  }
+}
+
+// The fixed-point weights that land on nodes with index n on some axis (a
+// particle in cell n - 1 of that axis), per species into the upper-face
+// arrays: the buckets of the tiles in the last tile layer of an axis (their
+// runs and the arrivals waiting in the neighbours' outboxes), and the spill
+// particles.  The same integer weights as the periodic deposit, so the
+// faces and the periodic accumulator split each particle's weight exactly.
+template <int T>
+__global__ __launch_bounds__(kBlock) void k_fx_faces(FxArgs a, unsigned long long *face,
+                                                     int64_t nf) {
+  __shared__ int64_t seg_src[kCodes];
+  __shared__ int seg_pre[kPre];
+  const int bid = blockIdx.x;
+  const Grid &g = a.g;
+  auto add = [&](int s, const Pcl &p) {
+    const int ix = min(p.cell & kCellMask, g.nx - 1),
+              iy = min((p.cell >> kCellBits) & kCellMask, g.ny - 1),
+              iz = min((int)((unsigned)p.cell >> (2 * kCellBits)), g.nz - 1);
+    if (ix < g.nx - 1 && iy < g.ny - 1 && iz < g.nz - 1) return;
+    int W[8];
+    fixed_weights(p.fx, p.fy, p.fz, W);
+    unsigned long long *f = face + (int64_t)s * nf;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int64_t fi = face_index(g, ix + (q & 1), iy + ((q >> 1) & 1), iz + (q >> 2));
+      if (fi >= 0) atomicAdd(&f[fi], (unsigned long long)(long long)W[q]);
+    }
+  };
+  if (bid < a.nb) {
+    const int b = bid;
+    const int s = b / a.ntiles, t = b - s * a.ntiles;
+    const int tx = t % a.ntx, ty = (t / a.ntx) % a.nty, tz = t / (a.ntx * a.nty);
+    if (tx != a.ntx - 1 && ty != a.nty - 1 && tz != a.ntz - 1) return;  // uniform
+    const int cnt = a.count_in[b];
+    const int nst = a.nst_in[b];
+    const int64_t beg = a.start_in[b];
+    const int64_t gap = a.cap_in[b] - cnt;
+    pull_table(a, b, seg_src, seg_pre);
+    __syncthreads();
+    const int narr = seg_pre[kCodes];
+    for (int i = threadIdx.x; i < cnt + narr; i += kBlock) {
+      Pcl p;
+      if (i < cnt) load_pcl(a.in, beg + i + (i >= nst ? gap : 0), p);
+      else load_rec(a.ob_in, arrival_rec(seg_src, seg_pre, i - cnt), p);
+      add(s, p);
+    }
+  } else {
+    const int k = bid - a.nb;
+    const int s = k / kSpillBlocksPerSpecies, sub = k % kSpillBlocksPerSpecies;
+    const int cnt = min(a.spill_in[s], (int32_t)a.spill_cap);
+    const int64_t beg = a.area + (int64_t)s * a.spill_cap;
+    for (int i = sub * kBlock + threadIdx.x; i < cnt; i += kBlock * kSpillBlocksPerSpecies) {
+      Pcl p;
+      load_pcl(a.in, beg + i, p);
+      add(s, p);
     }
   }
 }
@@ -1316,7 +1395,6 @@ FxArgs make_args(emg_ctx *c) {
   a.ob_out = B.ob[out];
   a.area = B.area;
   a.spill_cap = B.spill_cap;
-  a.nrec = (int32_t)(B.cap_total * 4);
   a.node32 = c->node32;
   a.rho = c->rhofx;
   a.jcur = c->jfx;
@@ -1441,9 +1519,10 @@ int fx_alloc(emg_ctx *c, int64_t ntotal) {
   B.spill_cap = align_up(maxs / 16 + 4096);
   if (B.spill_cap > (1 << 28)) B.spill_cap = 1 << 28;
   B.cap_total = B.area + B.spill_cap * c->sp.nspec;
-  if (B.cap_total >= (int64_t(1) << 29))  // masked stores: byte offsets below 2^31
-    return set_error(EMG_ERR_ARG, "FP32_FIXED holds at most 2^29 particle slots per context "
-                                  "(about 500M particles)");
+  // the step's capacity scan packs two sums into one 64-bit term (the first
+  // below 2^32); the hot loop's offsets are bucket-relative and do not bound it
+  if (B.cap_total >= (int64_t(1) << 32))
+    return set_error(EMG_ERR_ARG, "FP32_FIXED holds at most 2^32 particle slots per context");
   for (int k = 0; k < 2; k++) {
     StateFx &S = B.st[k];
     void *p[7];
@@ -1607,6 +1686,24 @@ void launch_fx_energy(emg_ctx *c) {
   hipLaunchKernelGGL(k_ke_final, dim3(c->sp.nspec), dim3(kBlock), 0, c->stream, c->ke_part,
                      B.ntiles, B.nb, 1, c->ke_out);
   timer_end(c, kTEnergy);
+}
+
+void launch_fx_faces(emg_ctx *c) {
+  FxBuckets &B = c->fx;
+  const int64_t nf = face_len(c->g);
+  launch_zero(c->face, sizeof(unsigned long long) * nf * c->sp.nspec, c->stream);
+  FxArgs a = make_args(c);
+  unsigned grid = (unsigned)(B.nb + c->sp.nspec * kSpillBlocksPerSpecies);
+  switch (B.tile) {
+    case 4:
+      hipLaunchKernelGGL(k_fx_faces<4>, dim3(grid), dim3(kBlock), 0, c->stream, a,
+                         (unsigned long long *)c->face, nf);
+      break;
+    case 8:
+      hipLaunchKernelGGL(k_fx_faces<8>, dim3(grid), dim3(kBlock), 0, c->stream, a,
+                         (unsigned long long *)c->face, nf);
+      break;
+  }
 }
 
 // sum of the fused push partials (one per bucket)

@@ -7,7 +7,9 @@ add_background_charge -> Poisson -> esfld2 -> energy.  ustep = 2, except the
 last step of a run (esses.F90:115-119).
 
 EM (src/main/emses.F90:345-573): bfield(1/2) -> grdcrt -> psolve1 (+ current
-fold) -> bfield(1/2) -> efield(-2 aj) -> charge (diagnostic).
+fold) -> bfield(1/2) -> efield(-2 aj) -> rhoupd (rho by continuity) ->
+charge(1, 1) (the per-species diagnostic rhodg; rho keeps its continuity
+update).
 
 The particle work runs in libemgpu through ctca_amd.emg; the field solve is
 the host side (ctca_amd.fieldsolve).
@@ -139,16 +141,24 @@ class EMRun:
         nz, ny, nx = self.n[2], self.n[1], self.n[0]
         self.ebc = np.zeros((nz, ny, nx, 6))
         self.tcs = 2.0 * su.cs
+        # rho(1, -1:n+1) and rhodg(2 nspec + 1, -1:n+1) of the reference
+        # (ohinit.F90:155-159, 189); rho from the initial charge (emses.F90
+        # initialisation), then by continuity
+        self.rho = np.zeros((nz + 3, ny + 3, nx + 3))
+        self.rhodg = np.zeros((2 * len(w.species) + 1, nz + 3, ny + 3, nx + 3))
+        self.ctx.charge(0, rho=self.rho)
 
     def step(self, ustep=2):
-        """emses main step for the fields that the particles see: B half step,
-        relocation, push + zigzag current, B half step, E full step."""
+        """emses main step: B half step, relocation, push + zigzag current,
+        B half step, E full step, rhoupd, charge(1, 1)."""
         self.ebc = fieldsolve.b_half(self.ebc)
         self.ctx.set_fields(fieldsolve.eb_halo(self.ebc[..., :3], self.ebc[..., 3:]))
         self.ctx.push_em(ustep)
         aj = self.ctx.current()
         self.ebc = fieldsolve.b_half(self.ebc)
         self.ebc = fieldsolve.e_full(self.ebc, aj, self.tcs)
+        self.rho = fieldsolve.rhoupd(self.rho, aj)        # emses.F90:474
+        self.ctx.charge(1, rhodg=self.rhodg)               # emses.F90:475
         return aj
 
     def close(self):

@@ -262,6 +262,34 @@ class Context:
     def deposit_rho(self, func=0):
         _chk(lib().emg_deposit_rho(self.h, int(func)))
 
+    def charge(self, func, rho=None, rhodg=None):
+        """charge(ps, func) with the periodic fold into the caller's arrays,
+        as the reference leaves them (charge.F90:44-50, 59-143; the fold of
+        esses.F90:555-562 and 594-601): func 0 fills rho, func 2 rho and
+        rhodg, func 1 only rhodg and leaves rho as it was (the EM loop keeps
+        the continuity-updated rho of rhoupd, emses.F90:474-476).  rho is
+        [k, j, i] over -1..n+1, rhodg [component, k, j, i] over -1..n+1 with
+        component is (0-based) holding species is; the other components
+        (the reference's 2 nspec + 1) are zeroed like the reference zeroes
+        them."""
+        func = int(func)
+        if func not in (0, 1, 2):
+            raise ValueError("func must be 0, 1 or 2")
+        shape = (self.n[2] + 3, self.n[1] + 3, self.n[0] + 3)
+        if func in (0, 2) and (rho is None or rho.shape != shape):
+            raise ValueError(f"rho must have the shape {shape} (-1..n+1)")
+        if func in (1, 2) and (rhodg is None or rhodg.shape[1:] != shape
+                               or rhodg.shape[0] < self.nspec):
+            raise ValueError(f"rhodg must have the shape (>= nspec, *{shape})")
+        self.deposit_rho(func)
+        lo, hi = (-1, -1, -1), tuple(v + 1 for v in self.n)
+        if func in (0, 2):
+            rho[...] = self.rho(lo, hi)
+        if func in (1, 2):
+            rhodg[...] = 0.0
+            for k in range(self.nspec):
+                rhodg[k] = self.rho(lo, hi, species=k)
+
     def push_em(self, ustep):
         _chk(lib().emg_push_em(self.h, int(ustep)))
 

@@ -80,6 +80,19 @@ def e_full(ebc, aj, tcs):
     return out
 
 
+def rhoupd(rho, aj):
+    """rhoupd (src/field/rhoupd.F90:40-49): rho -= 2 div aj on the nodes
+    0..n of a rho[k, j, i] over -1..n+1 (the ghost planes are left as they
+    are), aj[k, j, i, 3] the folded current on the periodic nodes."""
+    jx, jy, jz = aj[..., 0], aj[..., 1], aj[..., 2]
+    div = (jx - np.roll(jx, 1, 2)) + (jy - np.roll(jy, 1, 1)) + (jz - np.roll(jz, 1, 0))
+    nz, ny, nx = div.shape
+    d = np.pad(div, ((0, 1), (0, 1), (0, 1)), mode="wrap")   # nodes 0..n, node n = node 0
+    out = rho.copy()
+    out[1:nz + 2, 1:ny + 2, 1:nx + 2] -= d * 2.0
+    return out
+
+
 def field_energy(ebc, cs):
     """energy.F90:52-70 over the staggered cells 0..n-1."""
     vol = float(np.prod(ebc.shape[:3]))

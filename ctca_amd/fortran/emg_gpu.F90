@@ -336,4 +336,39 @@ contains
     end do
   end function emg_error_message
 
+  ! The body of charge(ps, func) with the periodic fold, for the reference's
+  ! rho(1, -1:n+1, ..., ps) and rhodg(2 nspec + 1, -1:n+1, ..., ps)
+  ! (charge.F90:44-50, 59-143; esses.F90:555-562, 594-601): func 0 fills rho,
+  ! func 2 rho and rhodg(1:nspec), func 1 only rhodg and leaves rho as the
+  ! caller had it -- the EM loop calls charge(1, 1) right after rhoupd(1),
+  ! whose continuity-updated rho must survive (emses.F90:474-476).  rhodg is
+  ! zeroed first, as the reference zeroes it.
+  function emg_charge(ctx, func, nspec, n, rho, rhodg) result(rc)
+    type(c_ptr), intent(in) :: ctx
+    integer(c_int32_t), intent(in) :: func, nspec, n(3)
+    real(c_double), intent(inout) :: rho(-1:n(1) + 1, -1:n(2) + 1, -1:n(3) + 1)
+    real(c_double), intent(inout) :: rhodg(:, :, :, :)
+    integer(c_int) :: rc
+    integer(c_int32_t) :: lo(3), hi(3), is
+    real(c_double), allocatable :: tmp(:, :, :)
+    lo = -1
+    hi = n + 1
+    rc = emg_deposit_rho(ctx, func)
+    if (rc /= EMG_OK) return
+    if (func == 0 .or. func == 2) then
+      rc = emg_get_rho(ctx, rho, lo, hi)
+      if (rc /= EMG_OK) return
+    end if
+    if (func == 1 .or. func == 2) then
+      rhodg = 0d0
+      allocate(tmp(-1:n(1) + 1, -1:n(2) + 1, -1:n(3) + 1))
+      do is = 1, nspec
+        rc = emg_get_rhodg(ctx, is - 1_c_int32_t, tmp, lo, hi)
+        if (rc /= EMG_OK) exit
+        rhodg(is, :, :, :) = tmp
+      end do
+      deallocate(tmp)
+    end if
+  end function emg_charge
+
 end module emg_gpu

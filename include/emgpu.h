@@ -131,14 +131,25 @@ int emg_push_es(emg_ctx *ctx, int32_t ustep);
 
 /*
  * charge(ps, func) + oh3_exchange_borders(rho, CRH) + add_boundary_charge
- * (src/pcl2field/charge.F90:4-412, src/main/esses.F90:530-562): CIC charge
- * deposition with the periodic fold.  func = 0 or 2 (2 also keeps the
- * per-species densities that rhodg(is, ...) holds).
+ * (src/pcl2field/charge.F90:4-412, src/main/esses.F90:530-601): CIC charge
+ * deposition with the periodic fold, per species on the device.  func selects
+ * what the caller reads back, as in the reference: 0 rho (emg_get_rho), 2 rho
+ * and the per-species rhodg(is, ...) (emg_get_rhodg), 1 rhodg only -- the
+ * caller's rho is left alone (the EM loop calls charge(1, 1) right after
+ * rhoupd(1), whose continuity-updated rho must survive, emses.F90:474-476).
+ * emg_charge in ctca_amd/fortran/emg_gpu.F90 is that body for the
+ * reference's arrays.
  */
 int emg_deposit_rho(emg_ctx *ctx, int32_t func);
 
-/* Read the folded rho(1, lo:hi)^3 (Fortran layout, periodic values on every
- * index, ghosts included).  emg_get_rhodg reads species is (0-based). */
+/* Read the folded rho(1, lo:hi)^3 (Fortran layout) of the last deposit.
+ * Nodes 0..n hold the periodic fold (node n equals node 0).  The ghost
+ * planes -1 and n + 1 hold what the reference's border exchange and
+ * add_boundary_charge leave there (ohhelp3.c:1010-1031, charge.F90:265-409):
+ * the raw deposit of node n in plane -1 and of node 0 in plane n + 1, folded
+ * along z for y ghosts inside z 0..n and along y and z for x ghosts inside
+ * y, z 0..n.  Indices beyond -1..n+1 hold periodic images.  emg_get_rhodg
+ * reads species is (0-based), folded the same way (rhodg, CRD). */
 int emg_get_rho(emg_ctx *ctx, double *rho, const int32_t lo[3], const int32_t hi[3]);
 int emg_get_rhodg(emg_ctx *ctx, int32_t is, double *rho, const int32_t lo[3],
                   const int32_t hi[3]);

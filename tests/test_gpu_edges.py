@@ -120,9 +120,12 @@ def test_particles_on_box_faces():
 
 
 def test_fp32_capacity_limit_is_refused():
+    # the hot loop addresses a bucket's slots from the bucket's first slot, so
+    # more than 2^29 slots per context work (the 1G ramp runs in
+    # test_gpu_fullsize); the step's capacity scan bounds a context at 2^32
     with pytest.raises(emg.EmgError) as e:
-        emg.Context((64, 64, 64), Q, QM, RM, (1 << 29, 1 << 20), precision=emg.FP32_FIXED)
-    assert "2^29" in str(e.value)
+        emg.Context((64, 64, 64), Q, QM, RM, (1 << 31, 1 << 31), precision=emg.FP32_FIXED)
+    assert "2^32" in str(e.value)
 
 
 def test_failed_device_allocation_returns_an_error():

@@ -14,8 +14,10 @@ that hold at any size (the oracle is too slow for these particle counts):
   * no particle is lost or gained.
 
 Workloads: magnetised 128^3 with 2 species at 64 per cell (268,435,456
-particles), the EM two-stream 256x256x128 (295,279,002), and the densest
-z-slab 512x512x32 of the exponential ramp (474,679,216)."""
+particles), the EM two-stream 256x256x128 (295,279,002), the densest
+z-slab 512x512x32 of the exponential ramp (474,679,216), and the whole ramp
+512x512x256 with 8 per cell per species on average (1,073,741,824
+particles on one GPU, the two forms in turn)."""
 import numpy as np
 import pytest
 
@@ -49,8 +51,8 @@ def check_charge(rho, q, count):
     assert abs(total - q * count) <= 1e-12 * abs(q * count), (total, q * count)
 
 
-@pytest.mark.parametrize("workload", [units.mag128, units.ramp512_slab],
-                         ids=["mag128_268M", "ramp512_slab_475M"])
+@pytest.mark.parametrize("workload", [units.mag128, units.ramp512_slab, units.ramp512],
+                         ids=["mag128_268M", "ramp512_slab_475M", "ramp512_full_1G"])
 def test_es_step_fp32_against_fp64_full_size(workload):
     w, su, spec = setup(workload)
     nsp = len(w.species)

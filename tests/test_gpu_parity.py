@@ -147,25 +147,26 @@ def test_deposit_fp64_vs_oracle():
     s = SMALL
     n = s["n"]
     p = small_state()
+    lo, hi = (-1, -1, -1), tuple(v + 1 for v in n)
     with ctx_for(n, s["q"], s["qm"], s["rm"], s["counts"]) as ctx:
         ctx.upload(p, s["counts"])
         ctx.deposit_rho(2)
-        full = ctx.rho(lo=(-1, -1, -1), hi=tuple(v + 1 for v in n))
+        full = ctx.rho(lo=lo, hi=hi)
         per = [ctx.rho(species=k) for k in range(2)]
+        per_full = [ctx.rho(lo=lo, hi=hi, species=k) for k in range(2)]
     rho, rhodg = oracle.charge(p, s["counts"], s["q"], n, with_species=True)
     oracle.fold_rho(rho, n)
     oracle.fold_rho(rhodg, n, ncomp=2)
-    # nodes 0..n (both aliases of the periodic fold)
-    g = full[1:n[2] + 2, 1:n[1] + 2, 1:n[0] + 2]
-    o = rho[1:n[2] + 2, 1:n[1] + 2, 1:n[0] + 2]
-    scale = np.abs(o).max()
-    assert np.abs(g - o).max() <= 1e-12 * scale
-    # ghosts hold the periodic images
-    assert np.array_equal(full[0], full[n[2]])
-    assert np.array_equal(full[n[2] + 2], full[2])
+    # the whole rho(1, -1:n+1)^3: nodes 0..n folded, and the ghost planes as
+    # the border exchange + add_boundary_charge leave them (raw node n in
+    # ghost -1, raw node 0 in ghost n + 1), which differ from periodic images
+    scale = np.abs(rho).max()
+    assert np.abs(full - rho).max() <= 1e-12 * scale
+    assert np.abs(rho[0] - rho[n[2]]).max() > 1e-3 * scale
     for k in range(2):
         ok = oracle.rho_nodes(rhodg)[..., k]
         assert np.abs(per[k] - ok).max() <= 1e-12 * np.abs(ok).max()
+        assert np.abs(per_full[k] - rhodg[..., k]).max() <= 1e-12 * np.abs(rhodg[..., k]).max()
 
 
 def test_energy_fp64_vs_oracle():
@@ -232,6 +233,7 @@ def test_fp32_fixed_one_step_vs_fp64_oracle(tile):
     eb = periodic_eb(n, rng, 0.05, 0.01)
     b0 = (0.0, 0.0, 0.0125)
     p = mt_particles(n, counts, (0.05, 0.005))
+    lo, hi = (-1, -1, -1), tuple(v + 1 for v in n)
     with ctx_for(n, q, qm, rm, counts, precision=emg.FP32_FIXED, b0=b0, tile=tile,
                  track_pid=True) as ctx:
         ctx.upload(p, counts)
@@ -241,6 +243,7 @@ def test_fp32_fixed_one_step_vs_fp64_oracle(tile):
         ctx.set_fields(eb)
         ctx.push_es(2)
         rho32 = [ctx.rho(species=k) for k in range(2)]
+        full32 = [ctx.rho(lo=lo, hi=hi, species=k) for k in range(2)]
         g, cnt = ctx.download()
     o = by_pid(start)
     g = by_pid(g)
@@ -261,6 +264,8 @@ def test_fp32_fixed_one_step_vs_fp64_oracle(tile):
     for k in range(2):
         ok = oracle.rho_nodes(rhodg)[..., k]
         assert np.abs(rho32[k] - ok).max() <= 1e-6 * np.abs(ok).max(), k
+        # ghost planes included (upper-face fixed-point deposit)
+        assert np.abs(full32[k] - rhodg[..., k]).max() <= 1e-6 * np.abs(rhodg[..., k]).max(), k
 
 
 def test_fp32_fixed_multistep_with_tile_crossings():
@@ -288,6 +293,8 @@ def test_fp32_fixed_multistep_with_tile_crossings():
             ctx.push_es(2)
         ctx.deposit_rho(0)
         rho32 = [ctx.rho(species=k) for k in range(2)]
+        full32 = [ctx.rho(lo=(-1, -1, -1), hi=tuple(v + 1 for v in n), species=k)
+                  for k in range(2)]
         ek32 = ctx.energy()
         g, cnt = ctx.download()
     assert list(cnt) == list(counts)
@@ -313,6 +320,7 @@ def test_fp32_fixed_multistep_with_tile_crossings():
     for k in range(2):
         ok = oracle.rho_nodes(rhodg)[..., k]
         assert np.abs(rho32[k] - ok).max() <= 1e-5 * np.abs(ok).max(), k
+        assert np.abs(full32[k] - rhodg[..., k]).max() <= 1e-5 * np.abs(ok).max(), k
     ok = oracle.energy(o, counts, rm, np.array(n, dtype=float))
     assert np.allclose(ek32, ok, rtol=1e-5)
 

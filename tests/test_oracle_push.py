@@ -173,6 +173,51 @@ def test_charge_single_particle_hand_computed():
     assert np.allclose(r, expect, rtol=0, atol=1e-15)
 
 
+def test_fold_ghost_planes_single_particles():
+    """The ghost planes the border exchange (ohhelp3.c:1010-1031: node n into
+    ghost -1, node 0 into ghost n + 1) and add_boundary_charge
+    (charge.F90:265-409: z over every plane index, y inside z 0..n, x inside
+    y, z 0..n) leave, for two particles whose CIC weights are known: one in
+    the corner cell n - 1 at offsets (0.25, 0.5, 0.75), one at the centre of
+    cell 0 (weight 1/8 on each node of its cell)."""
+    n = (4, 5, 6)
+    nx, ny, nz = n
+    qc = 2.0
+    p = particles([[nx - 0.75, ny - 0.5, nz - 0.25], [0.5, 0.5, 0.5]], [[0, 0, 0], [0, 0, 0]])
+    rho = oracle.charge(p, [2], [qc], n)
+    oracle.fold_rho(rho, n)
+
+    def at(i, j, k):               # rho(1, i, j, k), indices -1..n+1
+        return rho[k + 1, j + 1, i + 1]
+    wa = {nx - 1: 0.75, nx: 0.25}  # corner particle, x weights by node
+    wb = {ny - 1: 0.5, ny: 0.5}
+    wc = {nz - 1: 0.25, nz: 0.75}
+
+    def raw(i, j, k):              # raw deposit at node (i, j, k), i in 0..n
+        v = qc * wa.get(i, 0.0) * wb.get(j, 0.0) * wc.get(k, 0.0)
+        if i in (0, 1) and j in (0, 1) and k in (0, 1):
+            v += qc / 8
+        return v
+    assert at(-1, -1, -1) == raw(nx, ny, nz)
+    assert at(nx + 1, ny + 1, nz + 1) == raw(0, 0, 0) == qc / 8
+    # z ghosts: raw values, not folded along x or y
+    assert at(nx - 1, ny, -1) == raw(nx - 1, ny, nz)
+    assert at(0, 0, -1) == raw(0, 0, nz) == 0.0
+    assert at(nx, ny, -1) == raw(nx, ny, nz) != 0.0
+    assert at(1, 1, nz + 1) == raw(1, 1, 0) == qc / 8
+    # a y ghost inside z 0..n is folded along z only
+    assert at(nx - 1, -1, nz) == raw(nx - 1, ny, nz) + raw(nx - 1, ny, 0)
+    assert at(0, -1, 0) == raw(0, ny, 0) + raw(0, ny, nz) == 0.0
+    assert at(1, ny + 1, 1) == raw(1, 0, 1) == qc / 8
+    # an x ghost inside y, z 0..n is folded along y and z
+    assert at(-1, ny, nz) == sum(raw(nx, b, c) for b in (0, ny) for c in (0, nz))
+    assert at(nx + 1, 0, 0) == sum(raw(0, b, c) for b in (0, ny) for c in (0, nz))
+    # nodes 0..n: the full periodic fold
+    assert math.isclose(at(0, 0, 0), sum(raw(a, b, c) for a in (0, nx) for b in (0, ny)
+                                         for c in (0, nz)), rel_tol=0, abs_tol=1e-15)
+    assert at(nx, ny, nz) == at(0, 0, 0)
+
+
 def test_charge_sum_and_translation():
     n = (8, 6, 4)
     rng = np.random.default_rng(5)
