@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -213,6 +214,33 @@ static TimerState *tstate(emg_ctx *c) {
 }
 }  // namespace emg
 
+// Staging of emg_get_rho_noalloc: a file-scope device array instead of a
+// per-call allocation.  Every GPU that runs the library holds its own copy
+// until the process ends; all calls on one GPU share it.  Host code uses the
+// device address hipGetSymbolAddress gives for the current device (taken
+// once per device, at the first emg_create on it), never the array's name.
+namespace emg {
+constexpr int64_t kScratchDoubles = int64_t(1) << 24;  // 128 MiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
+static std::mutex g_scratch_mu;
+static std::vector<double *> g_scratch_dev;  // per device id
+
+// the scratch address on the current device (`dev`)
+static int scratch_address(int dev, double **out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (dev < 0) return set_error(EMG_ERR_ARG, "device id");
+  if ((size_t)dev >= g_scratch_dev.size()) g_scratch_dev.resize((size_t)dev + 1, nullptr);
+  if (!g_scratch_dev[dev]) {
+    void *p = nullptr;
+    int rc = hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "scratch address");
+    if (rc) return rc;
+    g_scratch_dev[dev] = static_cast<double *>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return EMG_OK;
+}
+}  // namespace emg
+
 #define CTX_CHECK(c)                                                   \
   do {                                                                 \
     if (!(c)) return set_error(EMG_ERR_ARG, "null context");           \
@@ -250,6 +278,7 @@ int emg_create(const emg_config *cfg, emg_ctx **out) {
   new (reinterpret_cast<emg_timer_holder *>(c + 1)) emg_timer_holder();
   c->cfg = *cfg;
   int rc = hip_check(hipGetDevice(&c->device), "hipGetDevice");
+  if (!rc) rc = scratch_address(c->device, &c->scratch);
   if (rc) {
     emg_destroy(c);
     return rc;
@@ -828,6 +857,13 @@ int emg_get_rho(emg_ctx *c, double *rho, const int32_t lo[3], const int32_t hi[3
   CTX_CHECK(c);
   if (!c->rho_valid) return set_error(EMG_ERR_STATE, "rho is not deposited for the current positions");
   return read_grid(c, rho, lo, hi, 1, -1, false);
+}
+
+int emg_get_rho_noalloc(emg_ctx *c, double *rho, const int32_t lo[3], const int32_t hi[3]) {
+  CTX_CHECK(c);
+  if (!c->rho_valid) return set_error(EMG_ERR_STATE, "rho is not deposited for the current positions");
+  if (!c->scratch) return set_error(EMG_ERR_STATE, "no scratch address for this device");
+  return read_grid(c, rho, lo, hi, 1, -1, false, c->scratch, kScratchDoubles);
 }
 
 int emg_get_rhodg(emg_ctx *c, int32_t is, double *rho, const int32_t lo[3], const int32_t hi[3]) {

@@ -183,6 +183,7 @@ struct emg_ctx {
 
   int *dflag;       // device error flags
   int *hflag;       // pinned host mirror
+  double *scratch;  // this device's static staging array (emg_get_rho_noalloc)
 
   // adjoint window (FP64 form)
   emg::State64 lam;  // lambda_x in x, y, z; lambda_v in vx, vy, vz

@@ -32,7 +32,8 @@ EMG_STATE_SLOTS = 64  # saved particle states per context (emg_state_save)
 EXPORTS = (
     "emg_create", "emg_destroy", "emg_last_error", "emg_set_stream", "emg_synchronize",
     "emg_upload_particles", "emg_download_particles", "emg_particle_counts", "emg_set_fields",
-    "emg_positn", "emg_push_es", "emg_deposit_rho", "emg_get_rho", "emg_get_rhodg", "emg_energy",
+    "emg_positn", "emg_push_es", "emg_deposit_rho", "emg_get_rho", "emg_get_rho_noalloc",
+    "emg_get_rhodg", "emg_energy",
     "emg_push_em", "emg_get_current", "emg_sort", "emg_init_device", "emg_host_inipcl",
     "emg_enable_timing", "emg_kernel_stats", "emg_reset_kernel_stats",
     "emg_state_save", "emg_state_load", "emg_adj_reset", "emg_adj_step", "emg_adj_get_field",
@@ -95,6 +96,7 @@ def lib():
             "emg_push_es": [vp, C.c_int32],
             "emg_deposit_rho": [vp, C.c_int32],
             "emg_get_rho": [vp, vp, i32p, i32p],
+            "emg_get_rho_noalloc": [vp, vp, i32p, i32p],
             "emg_get_rhodg": [vp, C.c_int32, vp, i32p, i32p],
             "emg_energy": [vp, vp],
             "emg_push_em": [vp, C.c_int32],
@@ -359,6 +361,14 @@ class Context:
             _chk(lib().emg_get_rho(self.h, _ptr(out), _i3(lo), _i3(hi)))
         else:
             _chk(lib().emg_get_rhodg(self.h, int(species), _ptr(out), _i3(lo), _i3(hi)))
+        return out
+
+    def rho_noalloc(self, lo=None, hi=None):
+        """rho(1, lo:hi)^3 like rho(), read through emg_get_rho_noalloc (no
+        device allocation: the library's static staging array)."""
+        lo, hi = self._bounds(lo, hi)
+        out = np.zeros((hi[2] - lo[2] + 1, hi[1] - lo[1] + 1, hi[0] - lo[0] + 1))
+        _chk(lib().emg_get_rho_noalloc(self.h, _ptr(out), _i3(lo), _i3(hi)))
         return out
 
     def current(self, lo=None, hi=None, species=None):

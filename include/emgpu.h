@@ -154,6 +154,17 @@ int emg_get_rho(emg_ctx *ctx, double *rho, const int32_t lo[3], const int32_t hi
 int emg_get_rhodg(emg_ctx *ctx, int32_t is, double *rho, const int32_t lo[3],
                   const int32_t hi[3]);
 
+/* emg_get_rho without any device allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code that
+ * records launches into a HIP graph, tight serving loops, harnesses that
+ * forbid hipMalloc).  The same read-out as emg_get_rho, staged through a
+ * library-owned static device array of at most 2^24 values (128 MiB; each
+ * GPU that runs the library holds one until the process ends):
+ * (hi - lo + 1) products beyond that return EMG_ERR_CAPACITY.  The array is
+ * shared by every context on a GPU, so calls that overlap on one GPU (two
+ * host threads) race; sequential calls do not (each returns after its copy). */
+int emg_get_rho_noalloc(emg_ctx *ctx, double *rho, const int32_t lo[3], const int32_t hi[3]);
+
 /* energy (src/etc/energy.F90:30-48): engk[3*is + d] =
  * 0.5 rm(is) sum v_d^2 / (slx sly slz), FP64. */
 int emg_energy(emg_ctx *ctx, double *engk);
