@@ -63,13 +63,15 @@ sys.path.insert(0, ROOT)
 
 METRIC = "particle-steps/s push+deposit, FP32/fixed-pt vs FP64 GPU; adjoint/forward time"
 HBM_PEAK_GBS = 8000.0
-# The FP64 form re-sorts itself by cell before every 12th push (emg_config
+# The FP64 form re-sorts itself by cell before every 10th push (emg_config
 # sort_every): its LDS windows need cell-ordered runs, and thermal drift
 # spreads a run beyond its window within ~20 steps (at the reference's CPU
 # cadence, isort = 500, the deposit slows from 6 to 35 ms in 60 steps on the
-# 128^3 workload; every 10-16 steps gives the lowest time per step, sorts
-# included).  A sort only permutes the particles.
-FP64_SORT_EVERY = 12
+# 128^3 workload).  With the fused push + charge kernel, 48 steps on the
+# 128^3 workload cost 10.2, 9.9, 9.2, 9.4, 9.6 and 11.7 ms per step, sorts
+# included, at a cadence of 6, 8, 10, 12, 16 and 24.  A sort only permutes
+# the particles.
+FP64_SORT_EVERY = 10
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_fx_step_mag128_pmc.txt")
 
 
@@ -309,8 +311,9 @@ def slab_leg(steps32, steps64, warmup, tile_arg=0, workload=None):
             ctx.close()
         v64 = npart * steps64 / el64
         out["fp64"] = {"particle_steps_per_s": v64, "ms_per_step": el64 / steps64 * 1e3,
-                       "steps": steps64, "push_kernel_ms": kernel_ms(st64, emg.KERNEL_PUSH),
-                       "deposit_kernel_ms": kernel_ms(st64, emg.KERNEL_DEPOSIT),
+                       "steps": steps64,
+                       "push_deposit_kernel_ms": kernel_ms(st64, emg.KERNEL_PUSH),
+                       "deposit_launches": st64[emg.KERNEL_DEPOSIT][1],
                        "sort_ms": sort_ms, "sort_every": FP64_SORT_EVERY}
         out["fp32_over_fp64"] = out["fp32"]["particle_steps_per_s"] / v64
     return out
@@ -506,31 +509,42 @@ def adjoint_window(steps, checkpoint):
         g_vd, g_e0 = win.backward()
         win.ctx.synchronize()
         t2 = time.perf_counter()
-        # central differences at three step sizes: the loss is piecewise smooth
-        # (a particle crossing a cell face kinks it), so a large step averages
-        # over kinks and a tiny one meets round-off; the adjoint is the limit
-        # (steps relative to the parameter, and the absolute h = 1e-3)
-        fd_steps = [(f"rel {r:g}", r * abs(vd), r * abs(e0)) for r in (1e-4, 1e-6, 1e-8)]
-        fd_steps.append(("h 0.001", 1e-3, 1e-3))
-        fd = {}
-        for label, hv, he in fd_steps:
-            fd[label] = ((win.forward(vd + hv, e0) - win.forward(vd - hv, e0)) / (2 * hv),
-                         (win.forward(vd, e0 + he) - win.forward(vd, e0 - he)) / (2 * he))
+        # Finite differences, declared before the run and applied to both
+        # parameters alike: central differences at the configuration's
+        # h = 1e-3 (the headline FD figure, pass/fail against 1e-4), and the
+        # same quotient at each step of a fixed ladder relative to the
+        # parameter, every step reported, none selected.  The loss is only
+        # piecewise smooth (a particle that crosses a cell face kinks it) and
+        # h = 1e-3 moves E0 by 10%, so large steps carry kink and curvature
+        # terms and tiny ones meet the round-off of the FP64 atomics.
+        def cd(hv, he):
+            return ((win.forward(vd + hv, e0) - win.forward(vd - hv, e0)) / (2 * hv),
+                    (win.forward(vd, e0 + he) - win.forward(vd, e0 - he)) / (2 * he))
+        h = 1e-3
+        fd_h = cd(h, h)
+        ladder = {f"rel {r:g}": cd(r * abs(vd), r * abs(e0)) for r in (1e-4, 1e-5, 1e-6, 1e-7, 1e-8)}
     finally:
         win.close()
-    # per parameter, the step whose difference quotient lies closest
-    best_vd = min(fd, key=lambda r: abs(g_vd - fd[r][0]) / abs(fd[r][0]))
-    best_e0 = min(fd, key=lambda r: abs(g_e0 - fd[r][1]) / abs(fd[r][1]))
-    fd_vd, fd_e0 = fd[best_vd][0], fd[best_e0][1]
+
+    def errs(v):
+        return {"dL_dvd": v[0], "dL_dE0": v[1], "rel_err_vd": abs(g_vd - v[0]) / abs(v[0]),
+                "rel_err_E0": abs(g_e0 - v[1]) / abs(v[1])}
+    head = errs(fd_h)
     return {"workload": w.name, "particles": int(win.counts.sum()), "steps": steps,
             "checkpoint_every": checkpoint, "loss": loss,
             "forward_s": t1 - t0, "adjoint_s": t2 - t1, "adjoint_over_forward": (t2 - t1) / (t1 - t0),
             "dL_dvd": g_vd, "dL_dE0": g_e0,
-            "fd": {r: {"dL_dvd": v[0], "dL_dE0": v[1],
-                              "rel_err_vd": abs(g_vd - v[0]) / abs(v[0]),
-                              "rel_err_E0": abs(g_e0 - v[1]) / abs(v[1])} for r, v in fd.items()},
-            "fd_step_closest": {"vd": best_vd, "E0": best_e0},
-            "rel_err_vd": abs(g_vd - fd_vd) / abs(fd_vd), "rel_err_E0": abs(g_e0 - fd_e0) / abs(fd_e0),
+            "fd_protocol": "central differences at h = 1e-3 on both parameters (the "
+                           "configuration's); bar 1e-4 relative",
+            "fd_h_1e-3": head,
+            "rel_err_vd": head["rel_err_vd"], "rel_err_E0": head["rel_err_E0"],
+            "fd_bar": 1e-4,
+            "fd_pass": bool(head["rel_err_vd"] <= 1e-4 and head["rel_err_E0"] <= 1e-4),
+            "fd_ladder_relative_steps": {k: errs(v) for k, v in ladder.items()},
+            "fp64_reference_gradient": "tests/test_gpu_adjoint.py::"
+                                       "test_adjoint_window_64cubed_k50_full_size: the reverse "
+                                       "sweep equals torch.autograd through an independent FP64 "
+                                       "restatement to 1e-10",
             "precision": "FP64 particle path, FP64 torch grid operators",
             "note": "adjoint time includes recomputing every window's states"}
 
@@ -638,15 +652,18 @@ def main():
         pm, pn = st64[emg.KERNEL_PUSH]
         dm, dn = st64[emg.KERNEL_DEPOSIT]
         derr = [float(np.abs(rho32[k] - rho64[k]).max() / np.abs(rho64[k]).max()) for k in range(nsp)]
+        # one fused kernel (k_push_dep64: psolve2 + charge at the new
+        # positions); emg_deposit_rho after it launches nothing
+        pk = pm / max(pn, 1)
+        bytes64 = 96.0 * npart + 48.0 * (n[0] + 1) * (n[1] + 1) * (n[2] + 1) + 8.0 * ncell * nsp
         fp64 = {"particle_steps_per_s": v64, "ms_per_step": el64 / steps64 * 1e3, "steps": steps64,
-                "push_kernel_ms": pm / max(pn, 1), "deposit_kernel_ms": dm / max(dn, 1),
-                "push_gbs_algorithmic": (96.0 * npart + 48.0 * (n[0] + 1) * (n[1] + 1) * (n[2] + 1))
-                / (pm / max(pn, 1) * 1e-3) / 1e9,
-                "deposit_gbs_algorithmic": (24.0 * npart + 8.0 * ncell * nsp) / (dm / max(dn, 1) * 1e-3) / 1e9,
+                "push_deposit_kernel_ms": pk, "deposit_launches": dn,
+                "gbs_algorithmic": bytes64 / (pk * 1e-3) / 1e9,
+                "hbm_frac": bytes64 / (pk * 1e-3) / 1e9 / HBM_PEAK_GBS,
                 "fp32_over_fp64": v32 / v64,
                 "density_err_fp32_vs_fp64_one_step": derr,
                 "sort_ms": sort_ms, "sort_every": FP64_SORT_EVERY,
-                "note": "ms_per_step includes the sort before every 12th push"}
+                "note": "ms_per_step includes the sort before every 10th push"}
         log(f"[bench] fp64: {el64 / steps64 * 1e3:.3f} ms/step; fp32/fp64 = {v32 / v64:.2f}x; "
             f"density err {derr}")
 
@@ -654,8 +671,9 @@ def main():
     if not args.no_adjoint:
         adj = adjoint_window(args.adjoint_steps, 10)
         log(f"[bench] adjoint window: forward {adj['forward_s']:.2f} s, adjoint {adj['adjoint_s']:.2f} s "
-            f"(x{adj['adjoint_over_forward']:.2f}); grad rel err {adj['rel_err_vd']:.1e}, "
-            f"{adj['rel_err_E0']:.1e}")
+            f"(x{adj['adjoint_over_forward']:.2f}); central FD at h = 1e-3: rel err "
+            f"{adj['rel_err_vd']:.1e} (v_d), {adj['rel_err_E0']:.1e} (E0), within 1e-4: "
+            f"{adj['fd_pass']}")
 
     drift = None
     if not args.no_drift:

@@ -787,9 +787,9 @@ int emg_push_es(emg_ctx *c, int32_t ustep) {
   if (ustep < 0) return set_error(EMG_ERR_ARG, "ustep must be >= 0");
   if ((rc = sort_if_due(c))) return rc;
   if (c->cfg.precision == EMG_PLAIN_FP64) {
-    launch_push_es64(c, ustep);
+    launch_push_es64(c, ustep);  // push + charge at the new positions
     if (c->moves_after_sort < emg::kUnsorted) c->moves_after_sort++;
-    c->rho_valid = false;
+    c->rho_valid = true;
     c->faces_valid = false;
     c->ke_valid = false;
   } else {
@@ -807,13 +807,14 @@ int emg_deposit_rho(emg_ctx *c, int32_t func) {
   int rc = need_particles(c);
   if (rc) return rc;
   if (func < 0 || func > 2) return set_error(EMG_ERR_ARG, "func must be 0, 1 or 2");
-  if (c->cfg.precision == EMG_PLAIN_FP64) {
-    launch_deposit64(c);
-  } else if (!c->rho_valid) {
-    launch_fx_deposit(c);
+  if (!c->rho_valid) {  // a push deposits at its new positions itself
+    if (c->cfg.precision == EMG_PLAIN_FP64)
+      launch_deposit64(c);
+    else
+      launch_fx_deposit(c);
+    c->rho_valid = true;
+    c->faces_valid = false;
   }
-  c->rho_valid = true;
-  c->faces_valid = false;
   return hip_check(hipGetLastError(), "deposit launch");
 }
 

@@ -176,37 +176,7 @@ __global__ __launch_bounds__(kBlock) void k_positn64(State64 s, SpeciesTab sp, G
   s.z[m] = z;
 }
 
-// ---------------------------------------------------------------- psolve2
-__global__ __launch_bounds__(kBlock) void k_push_es64(State64 s, SpeciesTab sp,
-                                                      const double *__restrict__ node, Grid g,
-                                                      int ustep, int *flag) {
-  int is = blockIdx.y;
-  int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= sp.cnt[is]) return;
-  m += sp.off[is];
-  double gustep = (double)ustep;
-  double qmp = sp.qm[is] * gustep * 0.5;
-  double x = s.x[m], y = s.y[m], z = s.z[m];
-  double vx = s.vx[m], vy = s.vy[m], vz = s.vz[m];
-  domain_guard(x, y, z, g, flag);
-  int i = (int)x, j = (int)y, k = (int)z;  // int(), psolve2.F90:158-160
-  Cic w = cic(x, y, z, i, j, k);
-  double f[6];
-  gather6(node, g, w, f);
-  boris64(vx, vy, vz, f, qmp);
-  double xmove = vx * gustep, ymove = vy * gustep, zmove = vz * gustep;
-  x = x + xmove;
-  y = y + ymove;
-  z = z + zmove;
-  wrap_periodic(x, y, z, g, flag);
-  s.x[m] = x;
-  s.y[m] = y;
-  s.z[m] = z;
-  s.vx[m] = vx;
-  s.vy[m] = vy;
-  s.vz[m] = vz;
-}
-
+// ---------------------------------------------------------------- psolve2 + charge
 // ---------------------------------------------------------------- charge
 // charge.F90:59-83 + distribute_charge 197-210; rho[is][node mod n] += v_k q(is)
 // charge.F90:152-210 per particle; the particles of a block are a contiguous
@@ -329,6 +299,59 @@ __device__ __forceinline__ void run_window(const State64 &s, int64_t a, int64_t 
   }
 }
 
+// One particle's CIC weights (charge.F90:152-210) at (x, y, z): into the
+// block's LDS window when its eight nodes lie in it, else into the species'
+// rho r directly.
+struct Win {
+  int wx, wy, wz, ox, oy, oz;
+};
+
+__device__ __forceinline__ void deposit_win(unsigned long long *win, const Win &W, double *r,
+                                            const Grid &g, double x, double y, double z,
+                                            double qc) {
+  int i = (int)x, j = (int)y, k = (int)z;
+  Cic w = cic(x, y, z, i, j, k);
+  // window coordinates of the cell (cell n, of a position at L, is cell 0's image)
+  int li = wrapn(i - W.ox + g.nx, g.nx), lj = wrapn(j - W.oy + g.ny, g.ny),
+      lk = wrapn(k - W.oz + g.nz, g.nz);
+  int li1, lj1, lk1;
+  if (li < W.wx && lj < W.wy && lk < W.wz && win_next(li, W.wx, g.nx, li1) &&
+      win_next(lj, W.wy, g.ny, lj1) && win_next(lk, W.wz, g.nz, lk1)) {
+    auto at = [&](int a, int b, int c) { return win + (c * W.wy + b) * W.wx + a; };
+    auto fx = [](double v) { return (unsigned long long)__double2ll_rn(v * kRhoFix); };
+    atomicAdd(at(li, lj, lk1), fx(w.v1));
+    atomicAdd(at(li1, lj, lk1), fx(w.v2));
+    atomicAdd(at(li1, lj1, lk1), fx(w.v3));
+    atomicAdd(at(li, lj1, lk1), fx(w.v4));
+    atomicAdd(at(li, lj, lk), fx(w.v5));
+    atomicAdd(at(li1, lj, lk), fx(w.v6));
+    atomicAdd(at(li1, lj1, lk), fx(w.v7));
+    atomicAdd(at(li, lj1, lk), fx(w.v8));
+  } else {
+    int i1 = i + 1, j1 = j + 1, k1 = k + 1;
+    atomicAdd(r + cidx(g, i, j, k1), w.v1 * qc);
+    atomicAdd(r + cidx(g, i1, j, k1), w.v2 * qc);
+    atomicAdd(r + cidx(g, i1, j1, k1), w.v3 * qc);
+    atomicAdd(r + cidx(g, i, j1, k1), w.v4 * qc);
+    atomicAdd(r + cidx(g, i, j, k), w.v5 * qc);
+    atomicAdd(r + cidx(g, i1, j, k), w.v6 * qc);
+    atomicAdd(r + cidx(g, i1, j1, k), w.v7 * qc);
+    atomicAdd(r + cidx(g, i, j1, k), w.v8 * qc);
+  }
+}
+
+// the window's non-zero nodes into rho, one FP64 atomic each (after a barrier)
+__device__ __forceinline__ void flush_win(const unsigned long long *win, const Win &W, double *r,
+                                          const Grid &g, double qc) {
+  for (int q = threadIdx.x; q < W.wx * W.wy * W.wz; q += kBlock) {
+    const long long v = (long long)win[q];
+    if (v != 0) {
+      int li = q % W.wx, lj = (q / W.wx) % W.wy, lk = q / (W.wx * W.wy);
+      atomicAdd(r + cidx(g, W.ox + li, W.oy + lj, W.oz + lk), (double)v * (qc / kRhoFix));
+    }
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_deposit64(State64 s, SpeciesTab sp, Grid g,
                                                       double *__restrict__ rho, int64_t ncell,
                                                       int *flag) {
@@ -340,53 +363,76 @@ __global__ __launch_bounds__(kBlock) void k_deposit64(State64 s, SpeciesTab sp, 
   const int64_t end = min(beg + (int64_t)kBlock * kDepPer, cnt);
   const int64_t off = sp.off[is];
   // window: nodes o .. o + w - 1 per axis (w <= n, so no node appears twice)
-  int wx, wy, wz, ox, oy, oz;
-  run_window(s, off + beg, off + end - 1, g, kWinCap, kWinX, kWinY, kWinZ, wx, wy, wz, ox, oy, oz);
-  for (int q = threadIdx.x; q < wx * wy * wz; q += kBlock) win[q] = 0ull;
+  Win W;
+  run_window(s, off + beg, off + end - 1, g, kWinCap, kWinX, kWinY, kWinZ, W.wx, W.wy, W.wz, W.ox,
+             W.oy, W.oz);
+  for (int q = threadIdx.x; q < W.wx * W.wy * W.wz; q += kBlock) win[q] = 0ull;
   __syncthreads();
   const double qc = sp.q[is];
   double *r = rho + (int64_t)is * ncell;
   for (int64_t m = beg + threadIdx.x; m < end; m += kBlock) {
     double x = s.x[off + m], y = s.y[off + m], z = s.z[off + m];
     domain_guard(x, y, z, g, flag);
-    int i = (int)x, j = (int)y, k = (int)z;
-    Cic w = cic(x, y, z, i, j, k);
-    // window coordinates of the cell (cell n, of a position at L, is cell 0's image)
-    int li = wrapn(i - ox + g.nx, g.nx), lj = wrapn(j - oy + g.ny, g.ny),
-        lk = wrapn(k - oz + g.nz, g.nz);
-    int li1, lj1, lk1;
-    if (li < wx && lj < wy && lk < wz && win_next(li, wx, g.nx, li1) &&
-        win_next(lj, wy, g.ny, lj1) && win_next(lk, wz, g.nz, lk1)) {
-      auto at = [&](int a, int b, int c) { return win + (c * wy + b) * wx + a; };
-      auto fx = [](double v) { return (unsigned long long)__double2ll_rn(v * kRhoFix); };
-      atomicAdd(at(li, lj, lk1), fx(w.v1));
-      atomicAdd(at(li1, lj, lk1), fx(w.v2));
-      atomicAdd(at(li1, lj1, lk1), fx(w.v3));
-      atomicAdd(at(li, lj1, lk1), fx(w.v4));
-      atomicAdd(at(li, lj, lk), fx(w.v5));
-      atomicAdd(at(li1, lj, lk), fx(w.v6));
-      atomicAdd(at(li1, lj1, lk), fx(w.v7));
-      atomicAdd(at(li, lj1, lk), fx(w.v8));
-    } else {
-      int i1 = i + 1, j1 = j + 1, k1 = k + 1;
-      atomicAdd(r + cidx(g, i, j, k1), w.v1 * qc);
-      atomicAdd(r + cidx(g, i1, j, k1), w.v2 * qc);
-      atomicAdd(r + cidx(g, i1, j1, k1), w.v3 * qc);
-      atomicAdd(r + cidx(g, i, j1, k1), w.v4 * qc);
-      atomicAdd(r + cidx(g, i, j, k), w.v5 * qc);
-      atomicAdd(r + cidx(g, i1, j, k), w.v6 * qc);
-      atomicAdd(r + cidx(g, i1, j1, k), w.v7 * qc);
-      atomicAdd(r + cidx(g, i, j1, k), w.v8 * qc);
-    }
+    deposit_win(win, W, r, g, x, y, z, qc);
   }
   __syncthreads();
-  for (int q = threadIdx.x; q < wx * wy * wz; q += kBlock) {
-    const long long v = (long long)win[q];
-    if (v != 0) {
-      int li = q % wx, lj = (q / wx) % wy, lk = q / (wx * wy);
-      atomicAdd(r + cidx(g, ox + li, oy + lj, oz + lk), (double)v * (qc / kRhoFix));
-    }
+  flush_win(win, W, r, g, qc);
+}
+
+// psolve2 and charge in one pass (psolve2.F90:137-433, then charge.F90:
+// 59-83): each particle of the block's run is pushed with the reference's
+// arithmetic in its order -- int() cell, CIC weights, eight-node gather,
+// Buneman-Boris, x += v ustep, periodic wrap -- and its charge deposited at
+// the new position exactly as k_deposit64 would.
+// The window is shaped from the run's positions before the move (read by
+// every thread before any is written: the barrier after zeroing it); the
+// particles that move out of it add to rho directly.
+__global__ __launch_bounds__(kBlock) void k_push_dep64(State64 s, SpeciesTab sp,
+                                                       const double *__restrict__ node, Grid g,
+                                                       int ustep, double *__restrict__ rho,
+                                                       int64_t ncell, int *flag) {
+  __shared__ unsigned long long win[kWinCap];
+  const int is = blockIdx.y;
+  const int64_t beg = (int64_t)blockIdx.x * kBlock * kDepPer;
+  const int64_t cnt = sp.cnt[is];
+  if (beg >= cnt) return;  // uniform over the block
+  const int64_t end = min(beg + (int64_t)kBlock * kDepPer, cnt);
+  const int64_t off = sp.off[is];
+  Win W;
+  run_window(s, off + beg, off + end - 1, g, kWinCap, kWinX, kWinY, kWinZ, W.wx, W.wy, W.wz, W.ox,
+             W.oy, W.oz);
+  for (int q = threadIdx.x; q < W.wx * W.wy * W.wz; q += kBlock) win[q] = 0ull;
+  __syncthreads();
+  const double gustep = (double)ustep;
+  const double qmp = sp.qm[is] * gustep * 0.5;
+  const double qc = sp.q[is];
+  double *r = rho + (int64_t)is * ncell;
+  for (int64_t mm = beg + threadIdx.x; mm < end; mm += kBlock) {
+    const int64_t m = off + mm;
+    double x = s.x[m], y = s.y[m], z = s.z[m];
+    double vx = s.vx[m], vy = s.vy[m], vz = s.vz[m];
+    domain_guard(x, y, z, g, flag);
+    int i = (int)x, j = (int)y, k = (int)z;  // int(), psolve2.F90:158-160
+    Cic w = cic(x, y, z, i, j, k);
+    double f[6];
+    gather6(node, g, w, f);
+    boris64(vx, vy, vz, f, qmp);
+    double xmove = vx * gustep, ymove = vy * gustep, zmove = vz * gustep;
+    x = x + xmove;
+    y = y + ymove;
+    z = z + zmove;
+    wrap_periodic(x, y, z, g, flag);
+    s.x[m] = x;
+    s.y[m] = y;
+    s.z[m] = z;
+    s.vx[m] = vx;
+    s.vy[m] = vy;
+    s.vz[m] = vz;
+    domain_guard(x, y, z, g, flag);
+    deposit_win(win, W, r, g, x, y, z, qc);
   }
+  __syncthreads();
+  flush_win(win, W, r, g, qc);
 }
 
 // ---------------------------------------------------------------- psolve1
@@ -625,9 +671,14 @@ void launch_positn64(emg_ctx *c, int ustep) {
 }
 
 void launch_push_es64(emg_ctx *c, int ustep) {
+  launch_zero(c->rho64, sizeof(double) * c->ncell * c->sp.nspec, c->stream);
   timer_begin(c, kTPush);
-  hipLaunchKernelGGL(k_push_es64, species_grid(c->sp), dim3(kBlock), 0, c->stream, c->s64, c->sp,
-                     c->node64, c->g, ustep, c->dflag);
+  int64_t mx = 1;
+  for (int q = 0; q < c->sp.nspec; q++) mx = c->sp.cnt[q] > mx ? c->sp.cnt[q] : mx;
+  dim3 grid((unsigned)((mx + (int64_t)kBlock * kDepPer - 1) / ((int64_t)kBlock * kDepPer)),
+            c->sp.nspec);
+  hipLaunchKernelGGL(k_push_dep64, grid, dim3(kBlock), 0, c->stream, c->s64, c->sp, c->node64,
+                     c->g, ustep, c->rho64, c->ncell, c->dflag);
   timer_end(c, kTPush);
 }
 

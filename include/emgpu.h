@@ -124,8 +124,8 @@ int emg_positn(emg_ctx *ctx, int32_t ustep);
 /*
  * psolve2(ps, ustep) (src/pcl2field/psolve2.F90:4-548): trilinear gather of
  * the relocated E/B, Buneman-Boris velocity update, x += v * ustep, periodic
- * wrap.  In EMG_FP32_FIXED the charge deposition at the new positions is done
- * in the same pass, so the following emg_deposit_rho costs nothing.
+ * wrap.  The charge deposition at the new positions is done in the same pass
+ * in both forms, so the following emg_deposit_rho costs nothing.
  */
 int emg_push_es(emg_ctx *ctx, int32_t ustep);
 
