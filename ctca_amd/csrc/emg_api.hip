@@ -295,6 +295,8 @@ int emg_create(const emg_config *cfg, emg_ctx **out) {
   c->g.slx = (double)cfg->n[0];
   c->g.sly = (double)cfg->n[1];
   c->g.slz = (double)cfg->n[2];
+  for (int b = 0; (1 << b) < cfg->n[0]; b++) c->g.cbx = b + 1;
+  for (int b = 0; (1 << b) < cfg->n[1]; b++) c->g.cby = b + 1;
   c->ncell = (int64_t)cfg->n[0] * cfg->n[1] * cfg->n[2];
   c->sp.nspec = cfg->nspec;
   int64_t off = 0;

@@ -16,8 +16,6 @@ constexpr int kFixShift = 24;  // rho fixed point: unit CIC weight = 2^24
 constexpr double kFixScale = 16777216.0;
 constexpr int kJShift = 24;  // current fixed point: 2^24 per unit flux (cell / unit time)
 constexpr double kJScale = 16777216.0;
-constexpr int kCellBits = 10;  // packed cell index: ix | iy << 10 | iz << 20
-constexpr int kCellMask = (1 << kCellBits) - 1;
 constexpr int kStateSlots = 64;
 constexpr int kUnsorted = 1 << 30;  // moves_after_sort of a state in no known order  // saved particle states of the adjoint window
 
@@ -31,6 +29,7 @@ enum TimerKind { kTPush = 0, kTDeposit = 1, kTSort = 2, kTRelocate = 3, kTEnergy
 struct Grid {
   int nx, ny, nz;
   double slx, sly, slz;
+  int cbx, cby;  // FP32 form: bits of ix and iy in the packed cell index (ceil(log2 n))
 };
 
 struct SpeciesTab {

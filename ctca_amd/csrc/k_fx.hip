@@ -62,8 +62,19 @@ constexpr uint32_t kNoStore = 0x80000000u;  // buffer offset past every bucket: 
 
 __device__ __forceinline__ int wrapc(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
-__device__ __forceinline__ int pack_cell(int ix, int iy, int iz) {
-  return ix | (iy << kCellBits) | (iz << (2 * kCellBits));
+// The packed cell index: ix in the low cbx bits, iy in the next cby, iz above
+// (the grid's bit widths, ceil(log2 n) for x and y; at most 32 bits in all).
+__device__ __forceinline__ int pack_cell(int ix, int iy, int iz, const Grid &g) {
+  return ix | (iy << g.cbx) | (iz << (g.cbx + g.cby));
+}
+__device__ __forceinline__ int cell_ix(int cell, const Grid &g) {
+  return cell & ((1 << g.cbx) - 1);
+}
+__device__ __forceinline__ int cell_iy(int cell, const Grid &g) {
+  return (cell >> g.cbx) & ((1 << g.cby) - 1);
+}
+__device__ __forceinline__ int cell_iz(int cell, const Grid &g) {
+  return (int)((unsigned)cell >> (g.cbx + g.cby));
 }
 
 struct FxArgs {
@@ -320,10 +331,9 @@ __device__ __forceinline__ void fixed_weights(float fx, float fy, float fz, int 
 }
 
 template <int T>
-__device__ __forceinline__ int tile_of_cell(int cell, int ntx, int nty) {
+__device__ __forceinline__ int tile_of_cell(int cell, const Grid &g, int ntx, int nty) {
   constexpr int kLog2T = T == 4 ? 2 : 3;
-  int ix = cell & kCellMask, iy = (cell >> kCellBits) & kCellMask,
-      iz = (int)((unsigned)cell >> (2 * kCellBits));
+  int ix = cell_ix(cell, g), iy = cell_iy(cell, g), iz = cell_iz(cell, g);
   return ((iz >> kLog2T) * nty + (iy >> kLog2T)) * ntx + (ix >> kLog2T);
 }
 
@@ -491,15 +501,15 @@ __device__ __forceinline__ void zig_seg_global(unsigned long long *j3, int64_t n
 // DEP: 0 no deposit, 1 charge (psolve2 + charge), 2 zigzag current (psolve1).
 template <int T, bool PUSH, int DEP, bool PID>
 __device__ __forceinline__ void fx_one(const FxArgs &a, int s, bool act, const Pcl &p, int ox,
-                                       int oy, int oz, bool lds_ok, const StateFx &outb,
+                                       int oy, int oz, int twx, int twy, int twz, bool lds_ok,
+                                       const StateFx &outb,
                                        int32_t nrec,
                                        int *n_stay, int *n_out, FxRec *outbox,
                                        unsigned char *ocode, const float4 *fA, const float2 *fB,
                                        unsigned long long *rh, double ke[3]) {
   constexpr int N1 = T + 1, N3 = T + 3;
   const Grid &g = a.g;
-  int ix = p.cell & kCellMask, iy = (p.cell >> kCellBits) & kCellMask,
-      iz = (int)((unsigned)p.cell >> (2 * kCellBits));
+  int ix = cell_ix(p.cell, g), iy = cell_iy(p.cell, g), iz = cell_iz(p.cell, g);
   int lx = ix - ox, ly = iy - oy, lz = iz - oz;  // cell in the staged tile
   bool in_tile = lds_ok && max((unsigned)lx, max((unsigned)ly, (unsigned)lz)) < (unsigned)T;
   float fx = p.fx, fy = p.fy, fz = p.fz, vx = p.vx, vy = p.vy, vz = p.vz;
@@ -546,7 +556,10 @@ __device__ __forceinline__ void fx_one(const FxArgs &a, int s, bool act, const P
   int rx = lx + dx + 1, ry = ly + dy + 1, rz = lz + dz + 1;
   unsigned rmax = max((unsigned)rx, max((unsigned)ry, (unsigned)rz));
   int rmin = min(rx, min(ry, rz));
-  bool stay = act && in_tile && rmin >= 1 && rmax <= (unsigned)T;
+  // the tile is twx x twy x twz cells (T, or less for a tile at the grid's
+  // upper face when n is not a multiple of T)
+  bool stay = act && in_tile && (unsigned)(rx - 1) < (unsigned)twx &&
+              (unsigned)(ry - 1) < (unsigned)twy && (unsigned)(rz - 1) < (unsigned)twz;
   ix = wrap_add(ix, dx, g.nx);
   iy = wrap_add(iy, dy, g.ny);
   iz = wrap_add(iz, dz, g.nz);
@@ -606,7 +619,7 @@ __device__ __forceinline__ void fx_one(const FxArgs &a, int s, bool act, const P
       }
     }
   }
-  const int cell = pack_cell(ix, iy, iz);
+  const int cell = pack_cell(ix, iy, iz, g);
   uint32_t off = kNoStore;
   if (m != 0) {
     sbase = __builtin_amdgcn_readlane(sbase, sleader);
@@ -638,15 +651,16 @@ __device__ __forceinline__ void fx_one(const FxArgs &a, int s, bool act, const P
       // direction of the new tile: the tile coordinate of each axis is 0, 1
       // or 2 for the tile before, the own tile or the tile after; moves
       // beyond a neighbouring tile are placed directly at the end
-      ocode[k] = rmax <= (unsigned)(T + 1)
-                     ? (unsigned char)(((rz == 0 ? 0 : (rz <= T ? 1 : 2)) * 3 +
-                                        (ry == 0 ? 0 : (ry <= T ? 1 : 2))) * 3 +
-                                       (rx == 0 ? 0 : (rx <= T ? 1 : 2)))
+      ocode[k] = (unsigned)rx <= (unsigned)(twx + 1) && (unsigned)ry <= (unsigned)(twy + 1) &&
+                         (unsigned)rz <= (unsigned)(twz + 1)
+                     ? (unsigned char)(((rz == 0 ? 0 : (rz <= twz ? 1 : 2)) * 3 +
+                                        (ry == 0 ? 0 : (ry <= twy ? 1 : 2))) * 3 +
+                                       (rx == 0 ? 0 : (rx <= twx ? 1 : 2)))
                      : (unsigned char)kFar;
     }
     if (__ballot(mover && !boxed)) {  // no outbox (spill rounds) or outbox full
       if (mover && !boxed) {
-        int dest = s * a.ntiles + tile_of_cell<T>(cell, a.ntx, a.nty);
+        int dest = s * a.ntiles + tile_of_cell<T>(cell, g, a.ntx, a.nty);
         place_direct(a, dest, cell, p.pid, fx, fy, fz, vx, vy, vz);
       }
     }
@@ -773,6 +787,7 @@ __global__ __launch_bounds__(T == 8 ? 512 : 256) void k_fx_step(FxArgs a) {
     s = b / a.ntiles;
     const int t = b - s * a.ntiles;
     const int ox = (t % a.ntx) * T, oy = ((t / a.ntx) % a.nty) * T, oz = (t / (a.ntx * a.nty)) * T;
+    const int twx = min(T, a.g.nx - ox), twy = min(T, a.g.ny - oy), twz = min(T, a.g.nz - oz);
     const int cnt = a.count_in[b];
     const int nst = a.nst_in[b];
     const uint32_t gap32 = (uint32_t)(a.cap_in[b] - cnt);
@@ -805,7 +820,8 @@ __global__ __launch_bounds__(T == 8 ? 512 : 256) void k_fx_step(FxArgs a) {
       }
     };
     auto one = [&](bool act, const Pcl &p) {
-      fx_one<T, PUSH, DEP, PID>(a, s, act, p, ox, oy, oz, true, outb, nrec, &n_stay, &n_out,
+      fx_one<T, PUSH, DEP, PID>(a, s, act, p, ox, oy, oz, twx, twy, twz, true, outb, nrec,
+                                &n_stay, &n_out,
                                     outbox, ocode, fA, fB, rh, ke);
     };
     // Software pipelined three deep: while a lane pushes particle k, the loads
@@ -925,7 +941,7 @@ __global__ __launch_bounds__(T == 8 ? 512 : 256) void k_fx_step(FxArgs a) {
         if (cq[k] < kFar) {
           a.ob_out[(int64_t)b * kObStride + opre[cq[k]] + rq[k]] = r;
         } else {
-          place_direct(a, s * a.ntiles + tile_of_cell<T>(r.cell, a.ntx, a.nty), r.cell, r.pid,
+          place_direct(a, s * a.ntiles + tile_of_cell<T>(r.cell, a.g, a.ntx, a.nty), r.cell, r.pid,
                        r.fx, r.fy, r.fz, r.vx, r.vy, r.vz);
         }
       }
@@ -974,8 +990,8 @@ __global__ __launch_bounds__(kBlock) void k_fx_deposit(FxArgs a) {
       Pcl p;
       if (i < cnt) load_pcl(a.in, beg + i + (i >= nst ? gap : 0), p);
       else load_rec(a.ob_in, arrival_rec(seg_src, seg_pre, i - cnt), p);
-      int ix = min(p.cell & kCellMask, g.nx - 1), iy = min((p.cell >> kCellBits) & kCellMask, g.ny - 1),
-          iz = min((int)((unsigned)p.cell >> (2 * kCellBits)), g.nz - 1);
+      int ix = min(cell_ix(p.cell, g), g.nx - 1), iy = min(cell_iy(p.cell, g), g.ny - 1),
+          iz = min(cell_iz(p.cell, g), g.nz - 1);
       int W[8];
       fixed_weights(p.fx, p.fy, p.fz, W);
       int rx = wrapc(ix - ox + 1, g.nx), ry = wrapc(iy - oy + 1, g.ny), rz = wrapc(iz - oz + 1, g.nz);
@@ -1009,8 +1025,8 @@ __global__ __launch_bounds__(kBlock) void k_fx_deposit(FxArgs a) {
     for (int i = sub * kBlock + threadIdx.x; i < cnt; i += kBlock * kSpillBlocksPerSpecies) {
       Pcl p;
       load_pcl(a.in, beg + i, p);
-      int ix = min(p.cell & kCellMask, g.nx - 1), iy = min((p.cell >> kCellBits) & kCellMask, g.ny - 1),
-          iz = min((int)((unsigned)p.cell >> (2 * kCellBits)), g.nz - 1);
+      int ix = min(cell_ix(p.cell, g), g.nx - 1), iy = min(cell_iy(p.cell, g), g.ny - 1),
+          iz = min(cell_iz(p.cell, g), g.nz - 1);
       int W[8];
       fixed_weights(p.fx, p.fy, p.fz, W);
       deposit_global(r, ix, iy, iz, W);
@@ -1032,9 +1048,8 @@ __global__ __launch_bounds__(kBlock) void k_fx_faces(FxArgs a, unsigned long lon
   const int bid = blockIdx.x;
   const Grid &g = a.g;
   auto add = [&](int s, const Pcl &p) {
-    const int ix = min(p.cell & kCellMask, g.nx - 1),
-              iy = min((p.cell >> kCellBits) & kCellMask, g.ny - 1),
-              iz = min((int)((unsigned)p.cell >> (2 * kCellBits)), g.nz - 1);
+    const int ix = min(cell_ix(p.cell, g), g.nx - 1), iy = min(cell_iy(p.cell, g), g.ny - 1),
+              iz = min(cell_iz(p.cell, g), g.nz - 1);
     if (ix < g.nx - 1 && iy < g.ny - 1 && iz < g.nz - 1) return;
     int W[8];
     fixed_weights(p.fx, p.fy, p.fz, W);
@@ -1118,9 +1133,9 @@ __global__ __launch_bounds__(kBlock) void k_fx_energy(FxArgs a) {
 template <int T>
 __device__ __forceinline__ int bucket_of_cell(int cell, int s, const Grid &g, int ntiles, int ntx,
                                               int nty) {
-  const int ix = min(cell & kCellMask, g.nx - 1), iy = min((cell >> kCellBits) & kCellMask, g.ny - 1),
-            iz = min((int)((unsigned)cell >> (2 * kCellBits)), g.nz - 1);
-  return s * ntiles + tile_of_cell<T>(pack_cell(ix, iy, iz), ntx, nty);
+  const int ix = min(cell_ix(cell, g), g.nx - 1), iy = min(cell_iy(cell, g), g.ny - 1),
+            iz = min(cell_iz(cell, g), g.nz - 1);
+  return s * ntiles + tile_of_cell<T>(pack_cell(ix, iy, iz, g), g, ntx, nty);
 }
 
 // The spill particles of the input buffer join the lists of the buckets their
@@ -1252,21 +1267,21 @@ __device__ __forceinline__ void range_load(const RangeDesc &d, const StateFx &in
 }
 
 template <int T>
-__global__ __launch_bounds__(kBlock) void k_fx_hist(RangeDesc d, StateFx in, int ntx, int nty,
-                                                    int32_t *hist) {
+__global__ __launch_bounds__(kBlock) void k_fx_hist(RangeDesc d, StateFx in, Grid g, int ntx,
+                                                    int nty, int32_t *hist) {
   int r = blockIdx.x / d.rb, sub = blockIdx.x % d.rb;
   int s;
   int64_t cnt = range_count(d, r, s);
   for (int64_t i = (int64_t)sub * kBlock + threadIdx.x; i < cnt; i += (int64_t)d.rb * kBlock) {
     Pcl p;
     range_load(d, in, r, i, p);
-    atomicAdd(&hist[s * d.ntiles + tile_of_cell<T>(p.cell, ntx, nty)], 1);
+    atomicAdd(&hist[s * d.ntiles + tile_of_cell<T>(p.cell, g, ntx, nty)], 1);
   }
 }
 
 template <int T>
 __global__ __launch_bounds__(kBlock) void k_fx_scatter(RangeDesc d, StateFx in, StateFx out,
-                                                       int ntx, int nty, const int64_t *start,
+                                                       Grid g, int ntx, int nty, const int64_t *start,
                                                        int32_t *cursor) {
   int r = blockIdx.x / d.rb, sub = blockIdx.x % d.rb;
   int s;
@@ -1274,7 +1289,7 @@ __global__ __launch_bounds__(kBlock) void k_fx_scatter(RangeDesc d, StateFx in, 
   for (int64_t i = (int64_t)sub * kBlock + threadIdx.x; i < cnt; i += (int64_t)d.rb * kBlock) {
     Pcl p;
     range_load(d, in, r, i, p);
-    int b = s * d.ntiles + tile_of_cell<T>(p.cell, ntx, nty);
+    int b = s * d.ntiles + tile_of_cell<T>(p.cell, g, ntx, nty);
     int64_t dst = start[b] + atomicAdd(&cursor[b], 1);
     store_pcl(out, dst, p.cell, p.pid, p.fx, p.fy, p.fz, p.vx, p.vy, p.vz);
   }
@@ -1309,7 +1324,7 @@ __global__ __launch_bounds__(kBlock) void k_fx_from64(State64 s64, SpeciesTab sp
     c[d] = ci;
     f[d] = fo;
   }
-  store_pcl(out, m, pack_cell(c[0], c[1], c[2]), s64.pid[m], f[0], f[1], f[2], (float)s64.vx[m],
+  store_pcl(out, m, pack_cell(c[0], c[1], c[2], g), s64.pid[m], f[0], f[1], f[2], (float)s64.vx[m],
             (float)s64.vy[m], (float)s64.vz[m]);
 }
 
@@ -1317,7 +1332,8 @@ __global__ __launch_bounds__(kBlock) void k_fx_from64(State64 s64, SpeciesTab sp
 // flat scan of counts)
 __global__ __launch_bounds__(kBlock) void k_fx_to64(StateFx in, const int64_t *bstart,
                                                     const int32_t *bcount, const int64_t *flat,
-                                                    int ntiles, SpeciesTab sp, State64 out) {
+                                                    int ntiles, SpeciesTab sp, Grid g,
+                                                    State64 out) {
   int b = blockIdx.x;
   int s = b / ntiles;
   int cnt = bcount[b];
@@ -1326,7 +1342,7 @@ __global__ __launch_bounds__(kBlock) void k_fx_to64(StateFx in, const int64_t *b
   for (int i = threadIdx.x; i < cnt; i += kBlock) {
     int64_t src = beg + i, dst = base + i;
     int cell = in.cell[src];
-    int ix = cell & kCellMask, iy = (cell >> kCellBits) & kCellMask, iz = cell >> (2 * kCellBits);
+    int ix = cell_ix(cell, g), iy = cell_iy(cell, g), iz = cell_iz(cell, g);
     out.x[dst] = (double)ix + (double)in.fx[src];
     out.y[dst] = (double)iy + (double)in.fy[src];
     out.z[dst] = (double)iz + (double)in.fz[src];
@@ -1440,12 +1456,12 @@ int bucket_ranges(emg_ctx *c, RangeDesc d, int nr, int rb, int src) {
   launch_zero(B.hist, sizeof(int32_t) * B.nb, c->stream);
   switch (B.tile) {
     case 4:
-      hipLaunchKernelGGL(k_fx_hist<4>, dim3(grid), dim3(kBlock), 0, c->stream, d, B.st[src], B.ntx,
-                         B.nty, B.hist);
+      hipLaunchKernelGGL(k_fx_hist<4>, dim3(grid), dim3(kBlock), 0, c->stream, d, B.st[src], c->g,
+                         B.ntx, B.nty, B.hist);
       break;
     case 8:
-      hipLaunchKernelGGL(k_fx_hist<8>, dim3(grid), dim3(kBlock), 0, c->stream, d, B.st[src], B.ntx,
-                         B.nty, B.hist);
+      hipLaunchKernelGGL(k_fx_hist<8>, dim3(grid), dim3(kBlock), 0, c->stream, d, B.st[src], c->g,
+                         B.ntx, B.nty, B.hist);
       break;
     default: return set_error(EMG_ERR_ARG, "tile must be 4 or 8");
   }
@@ -1457,11 +1473,11 @@ int bucket_ranges(emg_ctx *c, RangeDesc d, int nr, int rb, int src) {
   switch (B.tile) {
     case 4:
       hipLaunchKernelGGL(k_fx_scatter<4>, dim3(grid), dim3(kBlock), 0, c->stream, d, B.st[src],
-                         B.st[dst], B.ntx, B.nty, B.start[dst], B.cursor);
+                         B.st[dst], c->g, B.ntx, B.nty, B.start[dst], B.cursor);
       break;
     case 8:
       hipLaunchKernelGGL(k_fx_scatter<8>, dim3(grid), dim3(kBlock), 0, c->stream, d, B.st[src],
-                         B.st[dst], B.ntx, B.nty, B.start[dst], B.cursor);
+                         B.st[dst], c->g, B.ntx, B.nty, B.start[dst], B.cursor);
       break;
   }
   // all particles of a bucket form one run from its start; no outbox contents
@@ -1496,18 +1512,30 @@ RangeDesc bucket_desc(emg_ctx *c) {
 
 int fx_alloc(emg_ctx *c, int64_t ntotal) {
   FxBuckets &B = c->fx;
+  const Grid &g = c->g;
   int T = c->cfg.tile;
-  if (T <= 0)  // default: 8^3-cell tiles where the grid allows them
-    T = (c->g.nx % 8 == 0 && c->g.ny % 8 == 0 && c->g.nz % 8 == 0) ? 8 : 4;
+  if (T <= 0) {  // default: 8^3-cell tiles where every axis is a multiple of 8
+    const bool m8 = g.nx % 8 == 0 && g.ny % 8 == 0 && g.nz % 8 == 0;
+    const bool m4 = g.nx % 4 == 0 && g.ny % 4 == 0 && g.nz % 4 == 0;
+    const bool ge8 = g.nx >= 8 && g.ny >= 8 && g.nz >= 8;
+    T = m8 ? 8 : (m4 ? 4 : (ge8 ? 8 : 4));
+  }
   if (T != 4 && T != 8) return set_error(EMG_ERR_ARG, "tile must be 4 or 8");
-  if (c->g.nx % T || c->g.ny % T || c->g.nz % T)
-    return set_error(EMG_ERR_ARG, "grid must be a multiple of the tile edge");
-  if (c->g.nx > (1 << kCellBits) || c->g.ny > (1 << kCellBits) || c->g.nz > (1 << kCellBits))
-    return set_error(EMG_ERR_ARG, "FP32_FIXED supports at most 1024 cells per axis");
+  // a grid that is not a multiple of T ends in partial tiles; the tile and
+  // its halo span at most two periods of each axis
+  if (g.nx < T || g.ny < T || g.nz < T)
+    return set_error(EMG_ERR_UNSUPPORTED, "FP32_FIXED needs at least `tile` cells per axis "
+                                          "(4 with tile 4): use EMG_PLAIN_FP64 for this grid");
+  int bz = 0;
+  while ((1 << bz) < g.nz) bz++;
+  if (g.cbx + g.cby + bz > 32)
+    return set_error(EMG_ERR_UNSUPPORTED, "FP32_FIXED packs a cell index in 32 bits: ceil(log2 nx) "
+                                          "+ ceil(log2 ny) + ceil(log2 nz) must be at most 32; use "
+                                          "EMG_PLAIN_FP64 for this grid");
   B.tile = T;
-  B.ntx = c->g.nx / T;
-  B.nty = c->g.ny / T;
-  B.ntz = c->g.nz / T;
+  B.ntx = (g.nx + T - 1) / T;
+  B.nty = (g.ny + T - 1) / T;
+  B.ntz = (g.nz + T - 1) / T;
   B.ntiles = B.ntx * B.nty * B.ntz;
   B.nb = B.ntiles * c->sp.nspec;
   // every particle is in a bucket run, an outbox or the spill region, so the
@@ -1614,7 +1642,8 @@ int fx_to64(emg_ctx *c, State64 &dst) {
   if (rc) return rc;
   exclusive_scan_i32(B.count[B.cur], B.start[1 - B.cur], B.nb, c->scan_tmp, c->stream);
   hipLaunchKernelGGL(k_fx_to64, dim3(B.nb), dim3(kBlock), 0, c->stream, B.st[B.cur],
-                     B.start[B.cur], B.count[B.cur], B.start[1 - B.cur], B.ntiles, c->sp, dst);
+                     B.start[B.cur], B.count[B.cur], B.start[1 - B.cur], B.ntiles, c->sp, c->g,
+                     dst);
   return hip_check(hipGetLastError(), "fx_to64");
 }
 

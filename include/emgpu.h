@@ -65,7 +65,14 @@ typedef struct {
   int32_t nspec;       /* species count, <= EMG_MAX_SPECIES                  */
   int32_t precision;   /* EMG_PLAIN_FP64 or EMG_FP32_FIXED                   */
   int32_t tile;        /* FP32_FIXED: tile edge in cells, 4 or 8 (0 = 8 when
-                          every axis is a multiple of 8, else 4)             */
+                          every axis is a multiple of 8, 4 when a multiple
+                          of 4, else 8 if every axis has 8 cells or more,
+                          else 4).  Any grid with at least `tile` cells per
+                          axis and ceil(log2 nx) + ceil(log2 ny) +
+                          ceil(log2 nz) <= 32 (the packed cell index) is
+                          taken: tiles at the upper faces may be partial.
+                          Others return EMG_ERR_UNSUPPORTED (use
+                          EMG_PLAIN_FP64, which takes any grid)              */
   int32_t track_pid;   /* FP32_FIXED: carry pid through every move (+8 B per
                           particle-step); without it download returns pid 0  */
   int32_t sort_every;  /* PLAIN_FP64: re-sort by cell (emg_sort) before every
