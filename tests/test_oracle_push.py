@@ -269,6 +269,45 @@ def test_energy_closed_form():
             assert math.isclose(e[s, d], 0.5 * rm * float(np.sum(seg[f] ** 2)) / 64.0, rel_tol=1e-13)
 
 
+def test_energy_pinned_by_mechanics():
+    """Kinetic energy against what the mechanics fixes, not its own formula:
+    (1) hand values for particles of known speed (one particle per axis, so
+    a swapped component or a missing 1/2 or 1/V shows); (2) a push in a
+    uniform E with no B adds the closed-form kick dv = 2 qmp E = qm E ustep
+    to every velocity, so the energy after it is 1/2 rm sum (v + dv)^2 / V;
+    (3) a push in a uniform B with no E rotates each velocity about B, so
+    the energy is unchanged to round-off; (4) a particle with nid = -1 does
+    not count (energy.F90 skips it)."""
+    n = (4, 2, 8)
+    vol = 64.0
+    p = particles([[0.5, 0.5, 0.5], [1.5, 1.5, 1.5], [2.5, 0.5, 3.5]],
+                  [[3.0, 0.0, 0.0], [0.0, -2.0, 0.0], [0.0, 0.0, 0.5]])
+    e = oracle.energy(p, [3], [2.0], np.array(n, dtype=float))
+    assert e[0].tolist() == [0.5 * 2.0 * 9.0 / vol, 0.5 * 2.0 * 4.0 / vol, 0.5 * 2.0 * 0.25 / vol]
+    p["nid"][1] = -1
+    e = oracle.energy(p, [3], [2.0], np.array(n, dtype=float))
+    assert e[0, 1] == 0.0
+    # (2) uniform E: closed-form kick
+    rng = np.random.default_rng(17)
+    p = rand_particles(rng, n, 400, vth=0.1)
+    v0 = np.stack([p["vx"], p["vy"], p["vz"]], axis=1).copy()
+    E = np.array([0.02, -0.01, 0.005])
+    qm, rm, ustep = -1.0, 1.5, 2
+    wrk = oracle.grdcrt(uniform_eb(n, E=E), n, (0, 0, 0), (0, 0, 0))
+    oracle.psolve2(p, [400], [qm], wrk, n, ustep, np.array(n, dtype=float))
+    dv = qm * E * ustep
+    want = 0.5 * rm * np.sum((v0 + dv) ** 2, axis=0) / vol
+    e = oracle.energy(p, [400], [rm], np.array(n, dtype=float))
+    assert np.allclose(e[0], want, rtol=1e-12, atol=0)
+    # (3) uniform B: rotation keeps the energy
+    p = rand_particles(rng, n, 400, vth=0.1)
+    before = oracle.energy(p, [400], [rm], np.array(n, dtype=float)).sum()
+    wrk = oracle.grdcrt(uniform_eb(n, B=(0.03, -0.02, 0.05)), n, (0, 0, 0), (0, 0, 0))
+    oracle.psolve2(p, [400], [qm], wrk, n, ustep, np.array(n, dtype=float))
+    after = oracle.energy(p, [400], [rm], np.array(n, dtype=float)).sum()
+    assert math.isclose(after, before, rel_tol=1e-13)
+
+
 def test_chgntr_uniform_neutraliser():
     n = (6, 4, 5)
     q = [0.01, -0.02]
