@@ -398,6 +398,9 @@ int emg_destroy(emg_ctx *c) {
   dfree(c->ke_part);
   dfree(c->ke_out);
   dfree(c->face);
+  dfree(c->runs[0]);
+  dfree(c->runs[1]);
+  dfree(c->run_base);
   dfree(c->sort_count);
   dfree(c->sort_cursor);
   dfree(c->sort_start);
@@ -529,6 +532,7 @@ int emg_upload_particles(emg_ctx *c, const emg_particle *pbuf, const int64_t *co
   c->have_particles = true;
   c->pushes_since_sort = c->cfg.sort_every;  // unsorted: sort before the first push
   c->moves_after_sort = emg::kUnsorted;
+  c->runs_valid = false;
   c->rho_valid = false;
   c->faces_valid = false;
   c->j_valid = false;
@@ -989,6 +993,7 @@ int emg_state_load(emg_ctx *c, int32_t slot) {
   for (int s = 0; s < c->sp.nspec; s++) c->sp.cnt[s] = c->slot_cnt[slot][s];
   c->have_particles = true;
   c->moves_after_sort = emg::kUnsorted;
+  c->runs_valid = false;
   c->rho_valid = c->j_valid = c->ke_valid = false;
   c->faces_valid = false;
   return copy_state(c->slot[slot], c->s64, c->cap_total, c->stream);
@@ -1094,6 +1099,7 @@ int emg_init_device(emg_ctx *c, const emg_device_species *spec, int32_t nspec, u
   c->have_particles = true;
   c->pushes_since_sort = c->cfg.sort_every;  // unsorted: sort before the first push
   c->moves_after_sort = emg::kUnsorted;
+  c->runs_valid = false;
   c->rho_valid = false;
   c->faces_valid = false;
   c->j_valid = false;

@@ -98,6 +98,22 @@ struct Timer {
 
 struct EsSolver;  // device field solve (k_poisson.hip)
 
+// A block's run of the FP64 kernels after a sort: len consecutive particles
+// of species s from its segment's index beg.  Each species is cut into
+// blocks of `run` particles in the sort order (cells in z-major order); a
+// block whose particles span fewer than kRunSpan cells is one run, a longer
+// one (a sparse region) is cut into np = ceil(span / kRunCells) runs of
+// equal cell ranges, so that every run's LDS window covers its cells.
+struct Run {
+  int64_t beg;
+  int32_t len, s;
+};
+constexpr int kRunEs = 4096;   // particles per block of the charge / ES push kernels
+constexpr int kRunEm = 1024;   // of the zigzag push
+constexpr int kRunSpan = 96;   // cells a block may span and stay whole
+constexpr int kRunCells = 64;  // cells per run of a cut block
+static_assert(kRunSpan >= kRunCells, "a cut block's runs span over kRunCells / 2 cells");
+
 // Upper-face accumulators of the charge.  The periodic accumulators add the
 // weight a particle puts on node n of an axis into node 0; the reference
 // keeps it apart until its border exchange and add_boundary_charge, which
@@ -177,6 +193,11 @@ struct emg_ctx {
   int64_t *sort_start;
   int64_t sort_len;
   void *sort_rec;        // cap_total x 64-byte records + cap_total int32 ranks (FP64 sort, lazy)
+  emg::Run *runs[2];     // block runs of the last sort: [0] kRunEs, [1] kRunEm particles
+  int64_t nruns[2], runs_cap[2];
+  int64_t *run_base;     // scan scratch of the run counts (nspec x chunks + 1)
+  int64_t run_base_len;
+  bool runs_valid;       // the particles are in the order of the sort the runs describe
   int64_t *scan_tmp;
   int64_t scan_tmp_len;
 

@@ -448,6 +448,85 @@ dim3 species_grid(const SpeciesTab &sp) {
   return dim3((unsigned)((mx + kBlock - 1) / kBlock), (unsigned)sp.nspec, 1);
 }
 
+
+// Block runs after a sort.  st(c) = start[s nkey + c] - start[s nkey] is
+// the index (in the species' segment) of the first particle of cell c.
+struct RunCut {
+  const int64_t *st;  // start + s nkey
+  int64_t s0, nkey;
+  __device__ int64_t at(int64_t c) const { return st[c] - s0; }
+  // the cell of particle p: the largest c with st(c) <= p
+  __device__ int64_t cell_of(int64_t p) const {
+    int64_t lo = 0, hi = nkey - 1;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi + 1) >> 1;
+      if (at(mid) <= p) lo = mid;
+      else hi = mid - 1;
+    }
+    return lo;
+  }
+};
+
+// Block b of species s holds particles [a, e) = [b run, min((b + 1) run, cnt));
+// with cf, cl the cells of its first and last particle, it is one run if
+// cl - cf < kRunSpan, else np = ceil((cl - cf + 1) / kRunCells) pieces of
+// equal cell ranges (the first from a, the last to e; empty ones dropped).
+// visit(beg, end) is called for each run in order; returns the count.
+template <typename F>
+__device__ int64_t block_runs(const RunCut &rc, int64_t a, int64_t e, F visit) {
+  const int64_t cf = rc.cell_of(a), cl = rc.cell_of(e - 1);
+  const int64_t span = cl - cf + 1;
+  if (span <= kRunSpan) {
+    visit(a, e);
+    return 1;
+  }
+  const int64_t np = (span + kRunCells - 1) / kRunCells;
+  int64_t k = 0, pa = a;
+  for (int64_t q = 1; q <= np; q++) {
+    const int64_t pe = q == np ? e : rc.at(cf + q * span / np);
+    if (pe > pa) {
+      visit(pa, pe);
+      k++;
+      pa = pe;
+    }
+  }
+  return k;
+}
+
+// the number of runs per block (grid: blocks of species blockIdx.y) ...
+__global__ void k_run_count(const int64_t *start, int64_t nkey, SpeciesTab sp, int64_t nbmax,
+                            int run, int32_t *nrun) {
+  const int s = blockIdx.y;
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbmax) return;
+  const int64_t a = b * run, cnt = sp.cnt[s];
+  int32_t n = 0;
+  if (a < cnt) {
+    const RunCut rc{start + (int64_t)s * nkey, start[(int64_t)s * nkey], nkey};
+    n = (int32_t)block_runs(rc, a, min(a + run, cnt), [](int64_t, int64_t) {});
+  }
+  nrun[(int64_t)s * nbmax + b] = n;
+}
+
+// ... then, at each block's exclusive prefix, its runs
+__global__ void k_run_fill(const int64_t *start, int64_t nkey, SpeciesTab sp, int64_t nbmax,
+                           int run, const int64_t *base, Run *runs) {
+  const int s = blockIdx.y;
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbmax) return;
+  const int64_t a = b * run, cnt = sp.cnt[s];
+  if (a >= cnt) return;
+  const RunCut rc{start + (int64_t)s * nkey, start[(int64_t)s * nkey], nkey};
+  Run *out = runs + base[(int64_t)s * nbmax + b];
+  block_runs(rc, a, min(a + run, cnt), [&](int64_t p0, int64_t p1) {
+    Run r;
+    r.beg = p0;
+    r.len = (int32_t)(p1 - p0);
+    r.s = s;
+    *out++ = r;
+  });
+}
+
 }  // namespace
 
 void launch_zero(void *p, size_t bytes, hipStream_t s) {
@@ -607,11 +686,64 @@ int sort64(emg_ctx *c) {
     hipLaunchKernelGGL(k_sort_split, grid, dim3(kBlock), 0, c->stream,
                        (const uint4 *)c->sort_rec, c->t64, c->sp);
   }
+  // the blocks' runs in the new order, for both block lengths
+  c->runs_valid = false;
+  c->nruns[0] = c->nruns[1] = 0;
+  int rc = EMG_OK;
+  int64_t mx = 1;
+  for (int q = 0; q < c->sp.nspec; q++) mx = c->sp.cnt[q] > mx ? c->sp.cnt[q] : mx;
+  for (int k = 0; k < 2 && !rc; k++) {
+    const int run = k == 0 ? kRunEs : kRunEm;
+    const int64_t nbmax = (mx + run - 1) / run;
+    const int64_t nb = nbmax * c->sp.nspec;
+    // the counts reuse the sort's histogram (nkey nspec entries), which the
+    // scatter no longer reads; with more blocks than cells (over `run`
+    // particles per cell) the kernels keep their plain slices
+    if (nb > len) break;
+    if (c->run_base_len < nb + 1) {
+      if (c->run_base) (void)hipFree(c->run_base);
+      c->run_base_len = 0;
+      if (hipMalloc(&c->run_base, sizeof(int64_t) * (nb + 1)) != hipSuccess) {
+        c->run_base = nullptr;
+        rc = set_error(EMG_ERR_NOMEM, "sort runs");
+        break;
+      }
+      c->run_base_len = nb + 1;
+    }
+    // a cut block's pieces other than its last end at cell starts more than
+    // kRunCells / 2 cells apart (equal ranges of span / np cells, and
+    // span / np > span kRunCells / (span + kRunCells) >= kRunCells / 2 for
+    // span > kRunSpan >= kRunCells)
+    const int64_t cap = nb + (int64_t)c->sp.nspec * (nkey / (kRunCells / 2) + 2) + 1;
+    if (c->runs_cap[k] < cap) {
+      if (c->runs[k]) (void)hipFree(c->runs[k]);
+      c->runs_cap[k] = 0;
+      if (hipMalloc(&c->runs[k], sizeof(Run) * cap) != hipSuccess) {
+        c->runs[k] = nullptr;
+        rc = set_error(EMG_ERR_NOMEM, "sort runs");
+        break;
+      }
+      c->runs_cap[k] = cap;
+    }
+    const dim3 gr((unsigned)((nbmax + kBlock - 1) / kBlock), (unsigned)c->sp.nspec);
+    hipLaunchKernelGGL(k_run_count, gr, dim3(kBlock), 0, c->stream, c->sort_start, nkey, c->sp,
+                       nbmax, run, c->sort_count);
+    exclusive_scan_i32(c->sort_count, c->run_base, nb, c->scan_tmp, c->stream);
+    hipLaunchKernelGGL(k_run_fill, gr, dim3(kBlock), 0, c->stream, c->sort_start, nkey, c->sp,
+                       nbmax, run, c->run_base, c->runs[k]);
+    int64_t total = 0;
+    rc = hip_check(hipMemcpyAsync(&total, c->run_base + nb, sizeof(int64_t),
+                                  hipMemcpyDeviceToHost, c->stream), "sort runs");
+    if (!rc) rc = hip_check(hipStreamSynchronize(c->stream), "sort runs");
+    c->nruns[k] = total;
+  }
   timer_end(c, kTSort);
   c->moves_after_sort = 0;
   State64 tmp = c->s64;
   c->s64 = c->t64;
   c->t64 = tmp;
+  c->runs_valid = rc == EMG_OK && c->nruns[0] > 0 && c->nruns[1] > 0;
+  if (rc) return rc;
   return hip_check(hipGetLastError(), "sort");
 }
 

@@ -184,9 +184,9 @@ __global__ __launch_bounds__(kBlock) void k_positn64(State64 s, SpeciesTab sp, G
 // and they drift slowly, so most of a block's particles lie in a box of
 // nodes: the block sums their weights in an LDS window of that box, as int64
 // fixed point at 2^48 per unit weight (at most kBlock kDepPer terms of at
-// most 2^48 per node), and adds each non-zero window node times q(is) to rho
-// with one global FP64 atomic; a particle whose eight nodes are not all in
-// the window adds its weights to rho directly.  The rounding of a weight
+// most 2^48 per node, below 2^61), and adds each non-zero window node
+// times q(is) to rho with one global FP64 atomic; a particle whose eight
+// nodes are not all in the window adds its weights to rho directly.  The rounding of a weight
 // (2^-49) is below the FP64 round-off of the sequential sum, which the
 // reference's order would give.  The window's shape: run_window.
 constexpr int kDepPer = 16;
@@ -299,6 +299,27 @@ __device__ __forceinline__ void run_window(const State64 &s, int64_t a, int64_t 
   }
 }
 
+// The block's run of particles (indices in the species' segment): after a
+// sort the run table's entry (at most `per` particles over a bounded range
+// of cells), else the per-th slice of species blockIdx.y.  False for a block
+// past the end.
+__device__ __forceinline__ bool block_run(const SpeciesTab &sp, const Run *runs, int per, int &is,
+                                          int64_t &beg, int64_t &end) {
+  if (runs) {
+    const Run r = runs[blockIdx.x];
+    is = r.s;
+    beg = r.beg;
+    end = r.beg + r.len;
+    return true;
+  }
+  is = blockIdx.y;
+  beg = (int64_t)blockIdx.x * per;
+  const int64_t cnt = sp.cnt[is];
+  if (beg >= cnt) return false;
+  end = min(beg + (int64_t)per, cnt);
+  return true;
+}
+
 // One particle's CIC weights (charge.F90:152-210) at (x, y, z): into the
 // block's LDS window when its eight nodes lie in it, else into the species'
 // rho r directly.
@@ -354,13 +375,11 @@ __device__ __forceinline__ void flush_win(const unsigned long long *win, const W
 
 __global__ __launch_bounds__(kBlock) void k_deposit64(State64 s, SpeciesTab sp, Grid g,
                                                       double *__restrict__ rho, int64_t ncell,
-                                                      int *flag) {
+                                                      const Run *__restrict__ runs, int *flag) {
   __shared__ unsigned long long win[kWinCap];
-  const int is = blockIdx.y;
-  const int64_t beg = (int64_t)blockIdx.x * kBlock * kDepPer;
-  const int64_t cnt = sp.cnt[is];
-  if (beg >= cnt) return;  // uniform over the block
-  const int64_t end = min(beg + (int64_t)kBlock * kDepPer, cnt);
+  int is;
+  int64_t beg, end;
+  if (!block_run(sp, runs, kBlock * kDepPer, is, beg, end)) return;  // uniform over the block
   const int64_t off = sp.off[is];
   // window: nodes o .. o + w - 1 per axis (w <= n, so no node appears twice)
   Win W;
@@ -390,13 +409,12 @@ __global__ __launch_bounds__(kBlock) void k_deposit64(State64 s, SpeciesTab sp, 
 __global__ __launch_bounds__(kBlock) void k_push_dep64(State64 s, SpeciesTab sp,
                                                        const double *__restrict__ node, Grid g,
                                                        int ustep, double *__restrict__ rho,
-                                                       int64_t ncell, int *flag) {
+                                                       int64_t ncell, const Run *__restrict__ runs,
+                                                       int *flag) {
   __shared__ unsigned long long win[kWinCap];
-  const int is = blockIdx.y;
-  const int64_t beg = (int64_t)blockIdx.x * kBlock * kDepPer;
-  const int64_t cnt = sp.cnt[is];
-  if (beg >= cnt) return;  // uniform over the block
-  const int64_t end = min(beg + (int64_t)kBlock * kDepPer, cnt);
+  int is;
+  int64_t beg, end;
+  if (!block_run(sp, runs, kBlock * kDepPer, is, beg, end)) return;  // uniform over the block
   const int64_t off = sp.off[is];
   Win W;
   run_window(s, off + beg, off + end - 1, g, kWinCap, kWinX, kWinY, kWinZ, W.wx, W.wy, W.wz, W.ox,
@@ -475,8 +493,8 @@ __device__ __forceinline__ Relay relay_axis(double xnew, double lb, double cb, i
 // point at 2^48 per unit, and reach the global FP64 sums once per window
 // node.  A contribution is |vv w| <= 4 in the window (larger fluxes go
 // straight to the global sums), so a node's window sum of at most
-// 2 kBlock kEmPer terms stays below 2^61; the rounding of each term (2^-49)
-// is far below the FP64 summation round-off of the reference's order.
+// 2 kBlock kEmPer terms stays below 2^61; the rounding of each term
+// (2^-49) is far below the FP64 summation round-off of the reference's order.
 constexpr int kEmPer = 4;
 constexpr int kEmCap = 1600;                           // 3 x 12.8 KB of int64
 constexpr int kEmWinX = 40, kEmWinY = 8, kEmWinZ = 5;  // the fixed box
@@ -487,14 +505,13 @@ __device__ __forceinline__ long long em_fix(double v) { return __double2ll_rn(v 
 __global__ __launch_bounds__(kBlock) void k_push_em64(State64 s, SpeciesTab sp,
                                                       const double *__restrict__ node, Grid g,
                                                       int ustep, double *__restrict__ jacc,
-                                                      int64_t ncell, int *flag) {
+                                                      int64_t ncell, const Run *__restrict__ runs,
+                                                      int *flag) {
   constexpr int WN = kEmCap;
   __shared__ unsigned long long win[3 * WN];
-  const int is = blockIdx.y;
-  const int64_t beg = (int64_t)blockIdx.x * kBlock * kEmPer;
-  const int64_t cnt = sp.cnt[is];
-  if (beg >= cnt) return;  // uniform over the block
-  const int64_t end = min(beg + (int64_t)kBlock * kEmPer, cnt);
+  int is;
+  int64_t beg, end;
+  if (!block_run(sp, runs, kBlock * kEmPer, is, beg, end)) return;  // uniform over the block
   const int64_t off = sp.off[is];
   // window: nodes o .. o + w - 1 per axis (w <= n, so no node appears twice)
   int wx, wy, wz, ox, oy, oz;
@@ -670,39 +687,49 @@ void launch_positn64(emg_ctx *c, int ustep) {
   timer_end(c, kTPush);
 }
 
+// The grid of a kernel whose blocks take `per` particles each: the run table
+// `which` of the last sort when it is valid (one block per run), else
+// (slices of `per` particles, species) with runs = nullptr.
+static dim3 block_grid(const emg_ctx *c, int per, int which, const Run *&runs) {
+  static_assert(kBlock * kDepPer == kRunEs && kBlock * kEmPer == kRunEm,
+                "run lengths are the kernels' particles per block");
+  if (c->runs_valid && c->nruns[which] > 0) {
+    runs = c->runs[which];
+    return dim3((unsigned)c->nruns[which]);
+  }
+  runs = nullptr;
+  int64_t mx = 1;
+  for (int q = 0; q < c->sp.nspec; q++) mx = c->sp.cnt[q] > mx ? c->sp.cnt[q] : mx;
+  return dim3((unsigned)((mx + per - 1) / per), c->sp.nspec);
+}
+
 void launch_push_es64(emg_ctx *c, int ustep) {
   launch_zero(c->rho64, sizeof(double) * c->ncell * c->sp.nspec, c->stream);
   timer_begin(c, kTPush);
-  int64_t mx = 1;
-  for (int q = 0; q < c->sp.nspec; q++) mx = c->sp.cnt[q] > mx ? c->sp.cnt[q] : mx;
-  dim3 grid((unsigned)((mx + (int64_t)kBlock * kDepPer - 1) / ((int64_t)kBlock * kDepPer)),
-            c->sp.nspec);
+  const Run *runs;
+  const dim3 grid = block_grid(c, kBlock * kDepPer, 0, runs);
   hipLaunchKernelGGL(k_push_dep64, grid, dim3(kBlock), 0, c->stream, c->s64, c->sp, c->node64,
-                     c->g, ustep, c->rho64, c->ncell, c->dflag);
+                     c->g, ustep, c->rho64, c->ncell, runs, c->dflag);
   timer_end(c, kTPush);
 }
 
 void launch_deposit64(emg_ctx *c) {
   launch_zero(c->rho64, sizeof(double) * c->ncell * c->sp.nspec, c->stream);
   timer_begin(c, kTDeposit);
-  int64_t mx = 1;
-  for (int q = 0; q < c->sp.nspec; q++) mx = c->sp.cnt[q] > mx ? c->sp.cnt[q] : mx;
-  dim3 grid((unsigned)((mx + (int64_t)kBlock * kDepPer - 1) / ((int64_t)kBlock * kDepPer)),
-            c->sp.nspec);
+  const Run *runs;
+  const dim3 grid = block_grid(c, kBlock * kDepPer, 0, runs);
   hipLaunchKernelGGL(k_deposit64, grid, dim3(kBlock), 0, c->stream, c->s64, c->sp, c->g, c->rho64,
-                     c->ncell, c->dflag);
+                     c->ncell, runs, c->dflag);
   timer_end(c, kTDeposit);
 }
 
 void launch_push_em64(emg_ctx *c, int ustep) {
   launch_zero(c->j64, sizeof(double) * 3 * c->ncell * c->sp.nspec, c->stream);
   timer_begin(c, kTPush);
-  int64_t mx = 1;
-  for (int q = 0; q < c->sp.nspec; q++) mx = c->sp.cnt[q] > mx ? c->sp.cnt[q] : mx;
-  dim3 grid((unsigned)((mx + (int64_t)kBlock * kEmPer - 1) / ((int64_t)kBlock * kEmPer)),
-            c->sp.nspec);
+  const Run *runs;
+  const dim3 grid = block_grid(c, kBlock * kEmPer, 1, runs);
   hipLaunchKernelGGL(k_push_em64, grid, dim3(kBlock), 0, c->stream, c->s64, c->sp, c->node64,
-                     c->g, ustep, c->j64, c->ncell, c->dflag);
+                     c->g, ustep, c->j64, c->ncell, runs, c->dflag);
   timer_end(c, kTPush);
 }
 

@@ -532,6 +532,45 @@ def test_sort_every_only_permutes():
         ctx_for(n, s["q"], s["qm"], s["rm"], s["counts"], sort_every=-1)
 
 
+def test_sorted_runs_on_a_nonuniform_grid():
+    """After a sort the FP64 kernels take their blocks from the sort's run
+    table (chunks of consecutive cells, each cut into runs of nearly equal
+    length).  On a 30x26x22 grid (no multiple of the chunk sizes) whose
+    density falls steeply in z -- chunks of thousands of particles, chunks of
+    a few and empty ones -- sorting before every push leaves positions and
+    velocities bit-identical to a run that never sorts (matched by pid), the
+    densities and currents equal to round-off, and every particle's charge
+    in rho."""
+    n = (30, 26, 22)
+    counts = (300_000, 60_000)
+    q, qm, rm = (0.002, -0.002), (1.0, -0.01), (0.002, 0.2)
+    p = mt_particles(n, counts, (0.08, 0.01), seed=12)
+    # z = 0.8 nz u^4: hundreds per cell near z = 0, a few per cell near
+    # z = 0.8 nz and none above
+    p["z"] = 0.8 * n[2] * (p["z"] / n[2]) ** 4
+    rng = np.random.default_rng(8)
+    eb = periodic_eb(n, rng, 0.05, 0.02)
+    out = {}
+    for every in (0, 1):
+        with ctx_for(n, q, qm, rm, counts, b0=(0.0, 0.0, 0.03), sort_every=every) as ctx:
+            ctx.upload(p, counts)
+            ctx.set_fields(eb)
+            for _ in range(3):
+                ctx.push_es(2)
+            rho = [ctx.rho(species=k) for k in range(2)]
+            ctx.push_em(2)
+            j = ctx.current()
+            g, cnt = ctx.download()
+        out[every] = (by_pid(g), rho, j)
+    for f in FIELDS:
+        assert np.array_equal(out[0][0][f], out[1][0][f]), f
+    for k in range(2):
+        r0, r1 = out[0][1][k], out[1][1][k]
+        assert np.abs(r1 - r0).max() <= 1e-13 * np.abs(r0).max(), k
+        assert float(r1.sum()) == pytest.approx(q[k] * counts[k], rel=1e-12)
+    assert np.abs(out[1][2] - out[0][2]).max() <= 1e-12 * np.abs(out[0][2]).max()
+
+
 def test_push_em_fp64_vs_oracle():
     n = (12, 10, 8)
     counts = (20000, 5000)
