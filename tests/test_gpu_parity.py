@@ -571,6 +571,36 @@ def test_sorted_runs_on_a_nonuniform_grid():
     assert np.abs(out[1][2] - out[0][2]).max() <= 1e-12 * np.abs(out[0][2]).max()
 
 
+@pytest.mark.parametrize("count", [150_000, 400_000], ids=["es_runs_only", "no_runs"])
+def test_sorted_runs_fall_back_on_crowded_grids(count):
+    """More blocks of a run length than cells (over 1,024 or 4,096 particles
+    per cell) leave that run table unbuilt and the kernels on their plain
+    slices: 150,000 particles on 4x4x4 cells build the charge table only,
+    400,000 build neither.  Sorting before every push still changes nothing
+    but the order."""
+    n = (4, 4, 4)
+    counts = (count, count // 5)
+    q, qm, rm = (0.002, -0.002), (1.0, -0.01), (0.002, 0.2)
+    p = mt_particles(n, counts, (0.08, 0.01), seed=13)
+    eb = periodic_eb(n, np.random.default_rng(9), 0.05, 0.02)
+    out = {}
+    for every in (0, 1):
+        with ctx_for(n, q, qm, rm, counts, b0=(0.0, 0.0, 0.03), sort_every=every) as ctx:
+            ctx.upload(p, counts)
+            ctx.set_fields(eb)
+            for _ in range(2):
+                ctx.push_es(2)
+            rho = ctx.rho()
+            ctx.push_em(2)
+            j = ctx.current()
+            g, _ = ctx.download()
+        out[every] = (by_pid(g), rho, j)
+    for f in FIELDS:
+        assert np.array_equal(out[0][0][f], out[1][0][f]), f
+    assert np.abs(out[1][1] - out[0][1]).max() <= 1e-13 * np.abs(out[0][1]).max()
+    assert np.abs(out[1][2] - out[0][2]).max() <= 1e-12 * np.abs(out[0][2]).max()
+
+
 def test_push_em_fp64_vs_oracle():
     n = (12, 10, 8)
     counts = (20000, 5000)
