@@ -742,7 +742,8 @@ int sort64(emg_ctx *c) {
   State64 tmp = c->s64;
   c->s64 = c->t64;
   c->t64 = tmp;
-  c->runs_valid = rc == EMG_OK && c->nruns[0] > 0 && c->nruns[1] > 0;
+  // a table left unbuilt has no runs: its kernels keep their plain slices
+  c->runs_valid = rc == EMG_OK;
   if (rc) return rc;
   return hip_check(hipGetLastError(), "sort");
 }
