@@ -508,22 +508,24 @@ __global__ void k_run_count(const int64_t *start, int64_t nkey, SpeciesTab sp, i
   nrun[(int64_t)s * nbmax + b] = n;
 }
 
-// ... then, at each block's exclusive prefix, its runs
+// ... then, at each block's exclusive prefix, its runs (none written at or
+// past cap, which the count bounds; the host checks the total against it)
 __global__ void k_run_fill(const int64_t *start, int64_t nkey, SpeciesTab sp, int64_t nbmax,
-                           int run, const int64_t *base, Run *runs) {
+                           int run, const int64_t *base, Run *runs, int64_t cap) {
   const int s = blockIdx.y;
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nbmax) return;
   const int64_t a = b * run, cnt = sp.cnt[s];
   if (a >= cnt) return;
   const RunCut rc{start + (int64_t)s * nkey, start[(int64_t)s * nkey], nkey};
-  Run *out = runs + base[(int64_t)s * nbmax + b];
+  int64_t at = base[(int64_t)s * nbmax + b];
   block_runs(rc, a, min(a + run, cnt), [&](int64_t p0, int64_t p1) {
     Run r;
     r.beg = p0;
     r.len = (int32_t)(p1 - p0);
     r.s = s;
-    *out++ = r;
+    if (at < cap) runs[at] = r;
+    at++;
   });
 }
 
@@ -686,7 +688,9 @@ int sort64(emg_ctx *c) {
     hipLaunchKernelGGL(k_sort_split, grid, dim3(kBlock), 0, c->stream,
                        (const uint4 *)c->sort_rec, c->t64, c->sp);
   }
-  // the blocks' runs in the new order, for both block lengths
+  // the blocks' runs in the new order, for both block lengths.  The tables
+  // only speed the kernels up: one that cannot be allocated is left unbuilt
+  // (its kernels keep their plain slices) and the sort still succeeds.
   c->runs_valid = false;
   c->nruns[0] = c->nruns[1] = 0;
   int rc = EMG_OK;
@@ -704,8 +708,8 @@ int sort64(emg_ctx *c) {
       if (c->run_base) (void)hipFree(c->run_base);
       c->run_base_len = 0;
       if (hipMalloc(&c->run_base, sizeof(int64_t) * (nb + 1)) != hipSuccess) {
+        (void)hipGetLastError();
         c->run_base = nullptr;
-        rc = set_error(EMG_ERR_NOMEM, "sort runs");
         break;
       }
       c->run_base_len = nb + 1;
@@ -719,8 +723,8 @@ int sort64(emg_ctx *c) {
       if (c->runs[k]) (void)hipFree(c->runs[k]);
       c->runs_cap[k] = 0;
       if (hipMalloc(&c->runs[k], sizeof(Run) * cap) != hipSuccess) {
+        (void)hipGetLastError();
         c->runs[k] = nullptr;
-        rc = set_error(EMG_ERR_NOMEM, "sort runs");
         break;
       }
       c->runs_cap[k] = cap;
@@ -730,12 +734,12 @@ int sort64(emg_ctx *c) {
                        nbmax, run, c->sort_count);
     exclusive_scan_i32(c->sort_count, c->run_base, nb, c->scan_tmp, c->stream);
     hipLaunchKernelGGL(k_run_fill, gr, dim3(kBlock), 0, c->stream, c->sort_start, nkey, c->sp,
-                       nbmax, run, c->run_base, c->runs[k]);
+                       nbmax, run, c->run_base, c->runs[k], cap);
     int64_t total = 0;
     rc = hip_check(hipMemcpyAsync(&total, c->run_base + nb, sizeof(int64_t),
                                   hipMemcpyDeviceToHost, c->stream), "sort runs");
     if (!rc) rc = hip_check(hipStreamSynchronize(c->stream), "sort runs");
-    c->nruns[k] = total;
+    c->nruns[k] = total <= cap ? total : 0;
   }
   timer_end(c, kTSort);
   c->moves_after_sort = 0;
