@@ -106,10 +106,95 @@ def test_em_step_fp32_against_fp64_full_size():
     d = j32 - j64
     # the relay point of the zigzag jumps where an end point crosses a cell
     # face, so the few particles whose FP32 and FP64 end points straddle a
-    # face set the maximum; the RMS measures the arithmetic
+    # face set the maximum (pinned node by node in
+    # test_em_current_difference_is_bounded_by_face_straddlers); the RMS
+    # measures the arithmetic
     assert np.sqrt((d * d).mean() / (j64 * j64).mean()) <= 1e-5
     assert np.abs(d).max() <= 2e-3 * np.abs(j64).max()
     t32 = j32.reshape(-1, 3).sum(axis=0, dtype=np.float64)
     t64 = j64.reshape(-1, 3).sum(axis=0, dtype=np.float64)
     scale = np.abs(j64).reshape(-1, 3).sum(axis=0, dtype=np.float64)
     assert np.all(np.abs(t32 - t64) <= 1e-6 * scale), (t32, t64)
+
+
+
+def test_em_current_difference_is_bounded_by_face_straddlers():
+    """Where the FP32 and FP64 zigzag currents differ by more than FP32
+    arithmetic, a particle's path ends (or starts) on one side of a cell face
+    in one form and on the other side in the other form, and its relay point
+    moves (psolve1.F90:399-415).  Pinned node by node, one step from the same
+    2M particles on 32^3 cells, a tenth of them drifting along x, with no
+    field (free streaming), where 100,000 particles are placed so that their
+    FP64 path ends within 2e-7 cells of a y face (about half of them then end
+    on the other side in FP32):
+
+      |J32 - J64| <= A + sum over the candidate straddlers near the node of
+                     4 |q| (max_d |dx_d| + 1e-6) / ustep
+
+    A = 2e-5 of the largest node (FP32 arithmetic).  A candidate is a
+    particle whose FP64 start or end lies within 1e-5 cells of a face (the two
+    forms' positions differ by ~1e-7 cells).  Each of a particle's two
+    segments moves at most |dx_d| along axis d and puts at most |q| |dx_d| /
+    ustep on a node (psolve1.F90:717-789), in each form: so the forms differ
+    by at most 4 |q| max_d |dx_d| / ustep at the nodes of its start and end
+    cells (with a cell of slack for the other form's cells).  The full-size
+    EM test's bounds (RMS 1e-5, maximum 2e-3 of the largest node) rest on
+    this."""
+    n = (32, 32, 32)
+    counts = (1_800_000, 200_000)
+    q, qm, rm = (0.002, 0.002), (-1.0, -1.0), (1.0, 1.0)
+    N = sum(counts)
+    rng = np.random.default_rng(34)
+    p = np.zeros(N, dtype=emg.PARTICLE)
+    for f, L in zip(("x", "y", "z"), n):
+        p[f] = rng.uniform(0.0, L, N)
+    for f in ("vx", "vy", "vz"):
+        p[f] = rng.normal(0.0, 0.02, N)
+    p["vx"][counts[0]:] += 0.05                       # the beam: 0.1 cells per push along x
+    p["pid"] = np.arange(N)
+    p["spec"] = np.repeat([1, 2], counts)
+    ustep = 2
+    k = rng.choice(N, 100_000, replace=False)
+    face = np.clip(np.rint(p["y"][k]), 1, n[1] - 1)
+    p["y"][k] = face + rng.uniform(-2e-7, 2e-7, len(k)) - p["vy"][k] * ustep
+    eb = fieldsolve.eb_halo(np.zeros(n[::-1] + (3,)))
+    out = {}
+    for prec in (emg.FP32_FIXED, emg.PLAIN_FP64):
+        kw = dict(tile=8, track_pid=True) if prec == emg.FP32_FIXED else {}
+        with emg.Context(n, q, qm, rm, counts, precision=prec, **kw) as ctx:
+            ctx.upload(p, counts)
+            ctx.set_fields(eb)
+            ctx.push_em(ustep)
+            out[prec] = (ctx.current(), ctx.download()[0])
+    j32, j64 = out[emg.FP32_FIXED][0], out[emg.PLAIN_FP64][0]
+    p1 = out[emg.PLAIN_FP64][1]                        # the upload's order (no sort)
+    assert np.array_equal(p1["pid"], p["pid"])
+    d = np.abs(j32 - j64)
+    A = 2e-5 * np.abs(j64).max()
+    L3 = np.array(n, dtype=np.float64)
+    near = np.zeros(N, dtype=bool)
+    for f in ("x", "y", "z"):
+        for st in (p, p1):
+            fr = st[f] - np.floor(st[f])
+            near |= (fr < 1e-5) | (fr > 1.0 - 1e-5)
+    idx = np.nonzero(near)[0]
+    s = np.repeat(np.arange(2), counts)[idx]
+    dx = np.stack([p1[f][idx] - p[f][idx] for f in ("x", "y", "z")], axis=1)
+    dx = (dx + L3 / 2) % L3 - L3 / 2                    # across the periodic faces
+    amount = 4.0 * np.abs(np.asarray(q))[s] * (np.abs(dx).max(axis=1) + 1e-6) / ustep
+    lo = np.stack([np.floor(np.minimum(p[f][idx], p[f][idx] + dx[:, a]))
+                   for a, f in enumerate(("x", "y", "z"))], axis=1).astype(np.int64) - 1
+    B = np.zeros(n[::-1])
+    for ox in range(5):
+        for oy in range(5):
+            for oz in range(5):
+                np.add.at(B, ((lo[:, 2] + oz) % n[2], (lo[:, 1] + oy) % n[1],
+                              (lo[:, 0] + ox) % n[0]), amount)
+    over = d > A
+    print("candidates", len(idx), "nodes over A", int(over.sum()),
+          "max |d| / max |J64|", d.max() / np.abs(j64).max())
+    assert over.sum() > 100          # the straddlers are there and are seen
+    bad = d > A + B[..., None]
+    assert not bad.any(), np.argwhere(bad)[:5]
+    # and only near them: no node without a candidate exceeds the arithmetic bound
+    assert not (over & (B[..., None] == 0)).any()
