@@ -839,17 +839,36 @@ __global__ __launch_bounds__(T == 8 ? 512 : 256) void k_fx_step(FxArgs a) {
     }
     pull_table(a, b, seg_src, seg_pre);
     if (PUSH) {
-      for (int q = threadIdx.x; q < 2 * N1 * N1 * N1; q += NT) {
-        int nq = q >> 1;
-        int lx = nq % N1, ly = (nq / N1) % N1, lz = nq / (N1 * N1);
-        int gx = wrapc(ox + lx, a.g.nx), gy = wrapc(oy + ly, a.g.ny), gz = wrapc(oz + lz, a.g.nz);
-        const float4 *pp = reinterpret_cast<const float4 *>(
-            a.node32 + 8 * (((int64_t)gz * a.g.ny + gy) * a.g.nx + gx));
-        if (q & 1) {
-          float4 Bv = pp[1];
-          fB[nq] = make_float2(Bv.x, Bv.y);
-        } else {
-          fA[nq] = pp[0];
+      // the field tile: each thread loads whole nodes (E and Bx, then By, Bz),
+      // a batch of nodes' loads issued before any is waited for, then stored
+      // (one batch of all its nodes; one node at a time in the current
+      // kernel, whose registers are fuller)
+      constexpr int NN = N1 * N1 * N1;
+      constexpr int R = (NN + NT - 1) / NT;
+      constexpr int RB = DEP == 2 ? 1 : R;
+#pragma unroll
+      for (int r0 = 0; r0 < R; r0 += RB) {
+        float4 va[RB];
+        float2 vb[RB];
+#pragma unroll
+        for (int r = 0; r < RB; r++) {
+          const int nq = (int)threadIdx.x + (r0 + r) * NT;
+          if (nq < NN) {
+            const int lx = nq % N1, ly = (nq / N1) % N1, lz = nq / (N1 * N1);
+            const int gx = wrapc(ox + lx, a.g.nx), gy = wrapc(oy + ly, a.g.ny),
+                      gz = wrapc(oz + lz, a.g.nz);
+            const float *pn = a.node32 + 8 * (((int64_t)gz * a.g.ny + gy) * a.g.nx + gx);
+            va[r] = *reinterpret_cast<const float4 *>(pn);
+            vb[r] = *reinterpret_cast<const float2 *>(pn + 4);
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < RB; r++) {
+          const int nq = (int)threadIdx.x + (r0 + r) * NT;
+          if (nq < NN) {
+            fA[nq] = va[r];
+            fB[nq] = vb[r];
+          }
         }
       }
     }
