@@ -807,6 +807,8 @@ __global__ __launch_bounds__(T == 8 ? 512 : 256) void k_fx_step(FxArgs a) {
     // nothing
     int total = 0, narr = 0;
     const int64_t sp0 = a.spre[b];
+    // read before the barrier below: after it the load would stall the block
+    const int nspill = a.shist[b];
     auto ldx = [&](int i, Pcl &p) {
       const int j = i + (int)threadIdx.x;
       if (j < cnt) {
@@ -876,7 +878,7 @@ __global__ __launch_bounds__(T == 8 ? 512 : 256) void k_fx_step(FxArgs a) {
       for (int q = threadIdx.x; q < kTileNodes; q += NT) rh[q] = 0ull;
     __syncthreads();
     narr = seg_pre[kCodes];
-    total = cnt + narr + a.shist[b];
+    total = cnt + narr + nspill;
     const int nfull = (cnt / NT) * NT;
     int base = 0;
     for (; base + 3 * NT <= nfull; base += 3 * NT) {
