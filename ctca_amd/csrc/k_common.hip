@@ -43,16 +43,47 @@ __device__ int64_t block_excl_scan(int64_t v, int64_t *total) {
 
 // With a second input in2, each term is in + 2^32 in2: one pass scans both
 // (the first sum must stay below 2^32), and k_scan_final splits the result.
-__device__ __forceinline__ int64_t scan_term(const int32_t *in, const int32_t *in2, int64_t i) {
-  return (int64_t)in[i] + (in2 ? (int64_t)in2[i] * 4294967296ll : 0);
+// A thread's kScanPer terms, all loaded before any is used (a full run as
+// four 16-byte loads per input: base is a multiple of kScanPer and the
+// inputs are allocation bases, so they are aligned), zero past n.
+__device__ __forceinline__ void scan_load(const int32_t *in, const int32_t *in2, int64_t n,
+                                          int64_t base, int64_t t[kScanPer]) {
+  static_assert(kScanPer % 4 == 0, "whole 16-byte loads");
+  int32_t a[kScanPer], b[kScanPer];
+  if (base + kScanPer <= n) {
+#pragma unroll
+    for (int q = 0; q < kScanPer / 4; q++) {
+      const int4 u = reinterpret_cast<const int4 *>(in + base)[q];
+      a[4 * q] = u.x;
+      a[4 * q + 1] = u.y;
+      a[4 * q + 2] = u.z;
+      a[4 * q + 3] = u.w;
+      int4 w = make_int4(0, 0, 0, 0);
+      if (in2) w = reinterpret_cast<const int4 *>(in2 + base)[q];
+      b[4 * q] = w.x;
+      b[4 * q + 1] = w.y;
+      b[4 * q + 2] = w.z;
+      b[4 * q + 3] = w.w;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < kScanPer; k++) {
+      a[k] = base + k < n ? in[base + k] : 0;
+      b[k] = in2 && base + k < n ? in2[base + k] : 0;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kScanPer; k++) t[k] = (int64_t)a[k] + (int64_t)b[k] * 4294967296ll;
 }
 
 __global__ __launch_bounds__(kBlock) void k_scan_sums(const int32_t *in, const int32_t *in2,
                                                       int64_t n, int64_t *bsum) {
   int64_t base = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * kScanPer;
+  int64_t t[kScanPer];
+  scan_load(in, in2, n, base, t);
   int64_t s = 0;
-  for (int k = 0; k < kScanPer; k++)
-    if (base + k < n) s += scan_term(in, in2, base + k);
+#pragma unroll
+  for (int k = 0; k < kScanPer; k++) s += t[k];
   int64_t tot;
   block_excl_scan(s, &tot);
   if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
@@ -76,9 +107,11 @@ __global__ __launch_bounds__(kBlock) void k_scan_final(const int32_t *in, const 
                                                        int64_t n, const int64_t *bsum,
                                                        int64_t nblk, int64_t *out, int64_t *out2) {
   int64_t base = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * kScanPer;
+  int64_t t[kScanPer];
+  scan_load(in, in2, n, base, t);
   int64_t s = 0;
-  for (int k = 0; k < kScanPer; k++)
-    if (base + k < n) s += scan_term(in, in2, base + k);
+#pragma unroll
+  for (int k = 0; k < kScanPer; k++) s += t[k];
   int64_t tot;
   int64_t run = block_excl_scan(s, &tot) + bsum[blockIdx.x];
   auto put = [&](int64_t i, int64_t v) {
@@ -89,10 +122,11 @@ __global__ __launch_bounds__(kBlock) void k_scan_final(const int32_t *in, const 
       out[i] = v;
     }
   };
+#pragma unroll
   for (int k = 0; k < kScanPer; k++)
     if (base + k < n) {
       put(base + k, run);
-      run += scan_term(in, in2, base + k);
+      run += t[k];
     }
   if (blockIdx.x == 0 && threadIdx.x == 0) put(n, bsum[nblk]);
 }

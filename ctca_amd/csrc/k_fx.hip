@@ -1203,12 +1203,21 @@ __global__ void k_fx_caps(const int32_t *count_in, const int32_t *pre_in, const 
                           int ntx, int nty, int ntz, int ntiles) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b < nb) {
-    int arr = 0;
+    // the 26 neighbours' direction counts, all loaded before any is added
+    int lo[kCodes], hi[kCodes];
+#pragma unroll
     for (int c = 0; c < kCodes; c++) {
-      if (c == 13) continue;
-      int n = neighbour_bucket(b, 26 - c, ntx, nty, ntz, ntiles);
-      arr += pre_in[(int64_t)n * kPre + c + 1] - pre_in[(int64_t)n * kPre + c];
+      lo[c] = 0;
+      hi[c] = 0;
+      if (c != 13) {
+        const int n = neighbour_bucket(b, 26 - c, ntx, nty, ntz, ntiles);
+        lo[c] = pre_in[(int64_t)n * kPre + c];
+        hi[c] = pre_in[(int64_t)n * kPre + c + 1];
+      }
     }
+    int arr = 0;
+#pragma unroll
+    for (int c = 0; c < kCodes; c++) arr += hi[c] - lo[c];
     cap_out[b] = (int32_t)align_up(count_in[b] + arr + shist[b] + kExtra);
     extra[b] = 0;
   }
