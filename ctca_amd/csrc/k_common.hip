@@ -400,15 +400,19 @@ __global__ __launch_bounds__(kBlock) void k_sort_scatter_direct(State64 s, State
   int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (m >= sp.cnt[is]) return;
   m += sp.off[is];
+  // every load before the first store: the arrays of s and t may alias as
+  // far as the compiler knows, so a load after a store would wait for it
+  const double x = s.x[m], y = s.y[m], z = s.z[m], vx = s.vx[m], vy = s.vy[m], vz = s.vz[m];
+  const int32_t pid = s.pid[m], rk = rank[m];
   int64_t key = (int64_t)is * nkey + slab_key(s, m, g);
-  int64_t d = sp.off[is] + (start[key] - start[(int64_t)is * nkey]) + rank[m];
-  t.x[d] = s.x[m];
-  t.y[d] = s.y[m];
-  t.z[d] = s.z[m];
-  t.vx[d] = s.vx[m];
-  t.vy[d] = s.vy[m];
-  t.vz[d] = s.vz[m];
-  t.pid[d] = s.pid[m];
+  int64_t d = sp.off[is] + (start[key] - start[(int64_t)is * nkey]) + rk;
+  t.x[d] = x;
+  t.y[d] = y;
+  t.z[d] = z;
+  t.vx[d] = vx;
+  t.vy[d] = vy;
+  t.vz[d] = vz;
+  t.pid[d] = pid;
 }
 
 __global__ __launch_bounds__(kBlock) void k_sort_split(const uint4 *rec, State64 t, SpeciesTab sp) {
