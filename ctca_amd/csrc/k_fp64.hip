@@ -186,9 +186,10 @@ __global__ __launch_bounds__(kBlock) void k_positn64(State64 s, SpeciesTab sp, G
 // fixed point at 2^48 per unit weight (at most kBlock kDepPer terms of at
 // most 2^48 per node, below 2^61), and adds each non-zero window node
 // times q(is) to rho with one global FP64 atomic; a particle whose eight
-// nodes are not all in the window adds its weights to rho directly.  The rounding of a weight
-// (2^-49) is below the FP64 round-off of the sequential sum, which the
-// reference's order would give.  The window's shape: run_window.
+// nodes are not all in the window adds its weights to rho directly.  The
+// rounding of a weight (2^-49) is below the FP64 round-off of the sequential
+// sum, which the reference's order would give.  The window's shape:
+// run_window; the block's particles: block_run.
 constexpr int kDepPer = 16;
 constexpr int kWinCap = 3840;                    // 30 KB of int64
 constexpr int kWinX = 80, kWinY = 8, kWinZ = 6;  // the fixed box
