@@ -339,29 +339,39 @@ __device__ __forceinline__ int64_t slab_key(const State64 &s, int64_t m, const G
 
 // count the particles of each key; the count a particle finds is its rank
 // among the key's particles (kept for the scatter).  The input is the last
-// sort's order moved by a few pushes, so neighbouring lanes mostly hold the
-// same key: each run of equal keys in a wave takes its counts with one atomic
-// (its first lane adds the run's length) and ranks its lanes from the old
-// count, instead of one contended atomic per particle.
+// sort's order moved by a few pushes: a wave holds a few distinct keys, the
+// particles that crossed a face interleaved with those that stayed.  The wave
+// groups its lanes by key (one ballot per distinct key, no memory access),
+// then each group's first lane adds the group's size with one atomic -- all
+// groups' atomics in one instruction -- and the lanes rank themselves inside
+// their group from the count it returns.
 __global__ __launch_bounds__(kBlock) void k_sort_hist(State64 s, SpeciesTab sp, Grid g, int64_t nkey,
                                                       int32_t *count, int32_t *rank) {
   const int is = blockIdx.y;
   const int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  const bool act = m < sp.cnt[is];  // the whole wave takes part in the shuffles
+  const bool act = m < sp.cnt[is];  // the whole wave takes part in the ballots
   const int lane = __lane_id();
-  const int64_t key = act ? (int64_t)is * nkey + slab_key(s, m + sp.off[is], g) : -1 - lane;
-  const int64_t prev = __shfl_up(key, 1);
-  const bool head = lane == 0 || prev != key;
-  const unsigned long long heads = __ballot(head);
-  // the run's first lane: the highest head at or below this lane
-  const unsigned long long below = heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
-  const int first = 63 - __clzll((long long)below);
-  const unsigned long long above = lane == 63 ? 0ull : heads & (~0ull << (lane + 1));
-  const int end = above ? __ffsll((long long)above) - 1 : 64;
+  const int64_t key = act ? (int64_t)is * nkey + slab_key(s, m + sp.off[is], g) : -1;
+  unsigned long long todo = __ballot(act);
+  int lead = lane, size = 0, inner = 0;
+  while (todo) {  // uniform over the wave; one round per distinct key
+    const int l = __builtin_amdgcn_readfirstlane(__ffsll((long long)todo) - 1);
+    const int64_t k = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(key >> 32), l)
+                                 << 32) |
+                                (uint32_t)__builtin_amdgcn_readlane((int)key, l));
+    const unsigned long long grp = __ballot(key == k);  // k >= 0: inactive lanes never match
+    if (lane == l) size = __popcll(grp);
+    if ((grp >> lane) & 1ull) {
+      lead = l;
+      inner = __builtin_amdgcn_mbcnt_hi((uint32_t)(grp >> 32),
+                                        __builtin_amdgcn_mbcnt_lo((uint32_t)grp, 0));
+    }
+    todo &= ~grp;
+  }
   int base = 0;
-  if (act && head) base = atomicAdd(&count[key], end - lane);
-  base = __shfl(base, first);
-  if (act) rank[m + sp.off[is]] = base + (lane - first);
+  if (act && lead == lane) base = atomicAdd(&count[key], size);
+  base = __shfl(base, lead);
+  if (act) rank[m + sp.off[is]] = base + inner;
 }
 
 // The destinations of a sort are a random permutation: written straight into

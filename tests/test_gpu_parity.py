@@ -459,11 +459,22 @@ def test_fixed_point_rho_is_order_and_tile_independent():
     assert tot == pytest.approx(counts[0] * q[0], rel=1e-12)
 
 
+def _cell_key(seg, n):
+    return (np.floor(seg["z"]) * n[1] + np.floor(seg["y"])) * n[0] + np.floor(seg["x"])
+
+
 def test_sort_fp64_is_a_z_slab_ordered_permutation():
+    """A sort of a random order (the record pass) and a re-sort five pushes
+    later (the direct pass, whose histogram groups each wave's lanes by cell):
+    both leave every species segment in cell order (z-major, so floor(z) never
+    decreases: an order the reference's comparator accepts) and permute the
+    particles without changing a bit of them."""
     s = SMALL
     n = s["n"]
     p = small_state()
-    with ctx_for(n, s["q"], s["qm"], s["rm"], s["counts"]) as ctx:
+    rng = np.random.default_rng(5)
+    eb = periodic_eb(n, rng, 0.05, 0.02)
+    with ctx_for(n, s["q"], s["qm"], s["rm"], s["counts"], b0=(0.0, 0.0, 0.03)) as ctx:
         ctx.upload(p, s["counts"])
         ctx.deposit_rho(0)
         r0 = ctx.rho()
@@ -471,17 +482,28 @@ def test_sort_fp64_is_a_z_slab_ordered_permutation():
         ctx.deposit_rho(0)
         r1 = ctx.rho()
         g, cnt = ctx.download()
-    off = 0
-    for c in s["counts"]:
-        seg = g[off:off + c]
-        assert np.all(np.diff(np.floor(seg["z"])) >= 0)
-        # the reference comparator's property (psort-struct.c:15-20)
-        assert np.all(np.diff(seg["z"]) > -1.0)
-        off += c
-    assert np.array_equal(np.sort(g["pid"]), np.sort(p["pid"]))
-    gp, pp = by_pid(g), by_pid(p)
-    for f in FIELDS:
-        assert np.array_equal(gp[f], pp[f])
+        ctx.set_fields(eb)
+        for _ in range(5):
+            ctx.push_es(2)
+        moved, _ = ctx.download()
+        ctx.sort()
+        g2, _ = ctx.download()
+    for before, after in ((p, g), (moved, g2)):
+        off = 0
+        for c in s["counts"]:
+            seg = after[off:off + c]
+            assert np.all(np.diff(_cell_key(seg, n)) >= 0)
+            assert np.all(np.diff(np.floor(seg["z"])) >= 0)
+            # the reference comparator's property (psort-struct.c:15-20)
+            assert np.all(np.diff(seg["z"]) > -1.0)
+            # each species keeps its own particles
+            assert np.array_equal(np.sort(seg["pid"]), np.sort(before[off:off + c]["pid"]))
+            off += c
+        gp, pp = by_pid(after), by_pid(before)
+        for f in FIELDS:
+            assert np.array_equal(gp[f], pp[f])
+    # the pushes moved particles across cells, so the re-sort had work to do
+    assert not np.array_equal(_cell_key(moved, n), _cell_key(g2, n))
     assert np.abs(r1 - r0).max() <= 1e-13 * np.abs(r0).max()
 
 
