@@ -755,14 +755,25 @@ static int need_fields(emg_ctx *c) {
   return EMG_OK;
 }
 
-// PLAIN_FP64 with sort_every > 0: the cell sort that is due before a push
-static int sort_if_due(emg_ctx *c) {
+// PLAIN_FP64 with sort_every > 0: the cell sort that is due before a push.
+// With `carry` (a push that can move the particles into the new order
+// itself) a re-sort close to the last sort only counts the cells here and
+// *carry is set: the caller launches its push with the re-sort and then
+// calls sort64_commit.
+static int sort_if_due(emg_ctx *c, bool *carry = nullptr) {
+  if (carry) *carry = false;
   if (c->cfg.precision != EMG_PLAIN_FP64 || c->cfg.sort_every <= 0) return EMG_OK;
   if (c->pushes_since_sort < c->cfg.sort_every) {
     c->pushes_since_sort++;
     return EMG_OK;
   }
-  int rc = sort64(c);
+  int rc;
+  if (carry && emg::sort64_fusable(c)) {
+    rc = emg::sort64_rank(c);
+    if (!rc) *carry = true;
+  } else {
+    rc = sort64(c);
+  }
   if (!rc) c->pushes_since_sort = 1;
   return rc;
 }
@@ -791,9 +802,11 @@ int emg_push_es(emg_ctx *c, int32_t ustep) {
   if (!rc) rc = need_fields(c);
   if (rc) return rc;
   if (ustep < 0) return set_error(EMG_ERR_ARG, "ustep must be >= 0");
-  if ((rc = sort_if_due(c))) return rc;
+  bool carry;
+  if ((rc = sort_if_due(c, &carry))) return rc;
   if (c->cfg.precision == EMG_PLAIN_FP64) {
-    launch_push_es64(c, ustep);  // push + charge at the new positions
+    launch_push_es64(c, ustep, carry);  // push + charge at the new positions
+    if (carry && (rc = emg::sort64_commit(c))) return rc;
     if (c->moves_after_sort < emg::kUnsorted) c->moves_after_sort++;
     c->rho_valid = true;
     c->faces_valid = false;
@@ -909,9 +922,11 @@ int emg_push_em(emg_ctx *c, int32_t ustep) {
   if (!rc) rc = need_fields(c);
   if (rc) return rc;
   if (ustep < 1) return set_error(EMG_ERR_ARG, "ustep must be >= 1");
-  if ((rc = sort_if_due(c))) return rc;
+  bool carry;
+  if ((rc = sort_if_due(c, &carry))) return rc;
   if (c->cfg.precision == EMG_PLAIN_FP64) {
-    launch_push_em64(c, ustep);
+    launch_push_em64(c, ustep, carry);
+    if (carry && (rc = emg::sort64_commit(c))) return rc;
     if (c->moves_after_sort < emg::kUnsorted) c->moves_after_sort++;
     c->ke_valid = false;
   } else {

@@ -108,6 +108,26 @@ struct Run {
   int64_t beg;
   int32_t len, s;
 };
+// The FP64 sort key of a position: its cell in z-major order, each index the
+// int() of the coordinate clamped into the grid.
+__host__ __device__ inline int64_t slab_key_xyz(double x, double y, double z, const Grid &g) {
+  int k = (int)z, j = (int)y, i = (int)x;
+  k = k < 0 ? 0 : (k >= g.nz ? g.nz - 1 : k);
+  j = j < 0 ? 0 : (j >= g.ny ? g.ny - 1 : j);
+  i = i < 0 ? 0 : (i >= g.nx ? g.nx - 1 : i);
+  return ((int64_t)k * g.ny + j) * g.nx + i;
+}
+
+// Where a push that carries a re-sort writes particle m of species is: slot
+// off[is] + start[is nkey + key] - start[is nkey] + rank[m] of t, key being
+// the cell of its position before the move (what sort64_rank counted).
+struct SortDest {
+  State64 t;
+  const int64_t *start;
+  const int32_t *rank;
+  int64_t nkey;
+};
+
 constexpr int kRunEs = 4096;   // particles per block of the charge / ES push kernels
 constexpr int kRunEm = 1024;   // of the zigzag push
 constexpr int kRunSpan = 96;   // cells a block may span and stay whole
@@ -238,9 +258,11 @@ void exclusive_scan2_i32(const int32_t *in, int64_t *out, const int32_t *in2, in
 // kernels: FP64 mirror (k_fp64.hip)
 void launch_relocate64(emg_ctx *c, int elo[3], int eext[3]);
 void launch_positn64(emg_ctx *c, int ustep);
-void launch_push_es64(emg_ctx *c, int ustep);
+// with resort, the push also moves the particles into the order sort64_rank
+// has counted (into t64; sort64_commit then makes it the live state)
+void launch_push_es64(emg_ctx *c, int ustep, bool resort);
 void launch_deposit64(emg_ctx *c);
-void launch_push_em64(emg_ctx *c, int ustep);
+void launch_push_em64(emg_ctx *c, int ustep, bool resort);
 void launch_faces64(emg_ctx *c);  // c->face (zeroed) += upper-face charge
 // device Yee update and field energy on eb_dev (k_yee.hip)
 void launch_bfield(emg_ctx *c);
@@ -260,7 +282,16 @@ void launch_energy64(emg_ctx *c);
 void launch_read_rho(emg_ctx *c, int species, double *dst_dev, const int lo[3], const int hi[3]);
 void launch_read_current(emg_ctx *c, int species, double *dst_dev, const int lo[3], const int hi[3]);
 void launch_zero(void *p, size_t bytes, hipStream_t s);
+// The FP64 cell sort.  sort64 sorts the live state.  A re-sort close to the
+// last sort can ride on the next push instead: sort64_rank counts the
+// current cells (each key's start, each particle's rank in its key), the
+// push writes every particle to its sorted slot (SortDest), and sort64_commit
+// makes that the live state and builds the blocks' run tables of it.
 int sort64(emg_ctx *c);
+bool sort64_fusable(const emg_ctx *c);
+int sort64_rank(emg_ctx *c);
+SortDest sort64_dest(const emg_ctx *c);
+int sort64_commit(emg_ctx *c);
 int init_device(emg_ctx *c, const emg_device_species *spec, int nspec, uint64_t seed);
 
 // adjoint of the FP64 ES push (k_adj.hip)

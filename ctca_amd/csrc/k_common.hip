@@ -330,11 +330,7 @@ __global__ __launch_bounds__(kBlock) void k_read_current(ReadArgs r, const void 
 // gather from and deposit to a small box of nodes (the order inside a cell is
 // whatever the atomic cursors give).
 __device__ __forceinline__ int64_t slab_key(const State64 &s, int64_t m, const Grid &g) {
-  int k = (int)s.z[m], j = (int)s.y[m], i = (int)s.x[m];
-  k = k < 0 ? 0 : (k >= g.nz ? g.nz - 1 : k);
-  j = j < 0 ? 0 : (j >= g.ny ? g.ny - 1 : j);
-  i = i < 0 ? 0 : (i >= g.nx ? g.nx - 1 : i);
-  return ((int64_t)k * g.ny + j) * g.nx + i;
+  return slab_key_xyz(s.x[m], s.y[m], s.z[m], g);
 }
 
 // count the particles of each key; the count a particle finds is its rank
@@ -684,7 +680,15 @@ void launch_read_current(emg_ctx *c, int species, double *dst, const int lo[3], 
 // pushes after a sort up to which a re-sort writes the arrays directly
 constexpr int kResortDirect = 32;
 
-int sort64(emg_ctx *c) {
+namespace {
+
+int32_t *rank_buf(const emg_ctx *c) {
+  return reinterpret_cast<int32_t *>(static_cast<char *>(c->sort_rec) + 64 * (size_t)c->cap_total);
+}
+
+// the sort's keys and ranks: count each key's particles (a particle's rank
+// is the count it finds) and scan the counts into each key's start
+int sort64_count(emg_ctx *c) {
   int64_t nkey = c->ncell;
   int64_t len = nkey * c->sp.nspec;
   if (c->sort_len < len) {
@@ -720,25 +724,24 @@ int sort64(emg_ctx *c) {
     c->sort_rec = nullptr;
     return set_error(EMG_ERR_NOMEM, "sort records");
   }
-  int32_t *rank = reinterpret_cast<int32_t *>(static_cast<char *>(c->sort_rec) + 64 * (size_t)c->cap_total);
   launch_zero(c->sort_count, sizeof(int32_t) * len, c->stream);
-  dim3 grid = species_grid(c->sp);
-  timer_begin(c, kTSort);
-  hipLaunchKernelGGL(k_sort_hist, grid, dim3(kBlock), 0, c->stream, c->s64, c->sp, c->g, nkey,
-                     c->sort_count, rank);
+  hipLaunchKernelGGL(k_sort_hist, species_grid(c->sp), dim3(kBlock), 0, c->stream, c->s64, c->sp,
+                     c->g, nkey, c->sort_count, rank_buf(c));
   exclusive_scan_i32(c->sort_count, c->sort_start, len, c->scan_tmp, c->stream);
-  if (c->moves_after_sort <= kResortDirect) {
-    hipLaunchKernelGGL(k_sort_scatter_direct, grid, dim3(kBlock), 0, c->stream, c->s64, c->t64,
-                       c->sp, c->g, nkey, c->sort_start, rank);
-  } else {
-    hipLaunchKernelGGL(k_sort_scatter, grid, dim3(kBlock), 0, c->stream, c->s64,
-                       (uint4 *)c->sort_rec, c->sp, c->g, nkey, c->sort_start, rank);
-    hipLaunchKernelGGL(k_sort_split, grid, dim3(kBlock), 0, c->stream,
-                       (const uint4 *)c->sort_rec, c->t64, c->sp);
-  }
-  // the blocks' runs in the new order, for both block lengths.  The tables
-  // only speed the kernels up: one that cannot be allocated is left unbuilt
-  // (its kernels keep their plain slices) and the sort still succeeds.
+  return hip_check(hipGetLastError(), "sort");
+}
+
+// the sorted order (in t64) becomes the live state, and the blocks' runs of
+// it are built, for both block lengths, from the key starts.  The tables only
+// speed the kernels up: one that cannot be allocated is left unbuilt (its
+// kernels keep their plain slices) and the sort still succeeds.
+int sort64_swap_runs(emg_ctx *c) {
+  const int64_t nkey = c->ncell;
+  const int64_t len = nkey * c->sp.nspec;
+  c->moves_after_sort = 0;
+  State64 tmp = c->s64;
+  c->s64 = c->t64;
+  c->t64 = tmp;
   c->runs_valid = false;
   c->nruns[0] = c->nruns[1] = 0;
   int rc = EMG_OK;
@@ -748,9 +751,9 @@ int sort64(emg_ctx *c) {
     const int run = k == 0 ? kRunEs : kRunEm;
     const int64_t nbmax = (mx + run - 1) / run;
     const int64_t nb = nbmax * c->sp.nspec;
-    // the counts reuse the sort's histogram (nkey nspec entries), which the
-    // scatter no longer reads; with more blocks than cells (over `run`
-    // particles per cell) the kernels keep their plain slices
+    // the counts reuse the sort's histogram (nkey nspec entries), which no
+    // later pass reads; with more blocks than cells (over `run` particles
+    // per cell) the kernels keep their plain slices
     if (nb > len) break;
     if (c->run_base_len < nb + 1) {
       if (c->run_base) (void)hipFree(c->run_base);
@@ -789,15 +792,57 @@ int sort64(emg_ctx *c) {
     if (!rc) rc = hip_check(hipStreamSynchronize(c->stream), "sort runs");
     c->nruns[k] = total <= cap ? total : 0;
   }
-  timer_end(c, kTSort);
-  c->moves_after_sort = 0;
-  State64 tmp = c->s64;
-  c->s64 = c->t64;
-  c->t64 = tmp;
   // a table left unbuilt has no runs: its kernels keep their plain slices
   c->runs_valid = rc == EMG_OK;
   if (rc) return rc;
   return hip_check(hipGetLastError(), "sort");
+}
+
+}  // namespace
+
+int sort64(emg_ctx *c) {
+  timer_begin(c, kTSort);
+  int rc = sort64_count(c);
+  if (!rc) {
+    const int64_t nkey = c->ncell;
+    const dim3 grid = species_grid(c->sp);
+    if (c->moves_after_sort <= kResortDirect) {
+      hipLaunchKernelGGL(k_sort_scatter_direct, grid, dim3(kBlock), 0, c->stream, c->s64, c->t64,
+                         c->sp, c->g, nkey, c->sort_start, rank_buf(c));
+    } else {
+      hipLaunchKernelGGL(k_sort_scatter, grid, dim3(kBlock), 0, c->stream, c->s64,
+                         (uint4 *)c->sort_rec, c->sp, c->g, nkey, c->sort_start, rank_buf(c));
+      hipLaunchKernelGGL(k_sort_split, grid, dim3(kBlock), 0, c->stream,
+                         (const uint4 *)c->sort_rec, c->t64, c->sp);
+    }
+    rc = sort64_swap_runs(c);
+  }
+  timer_end(c, kTSort);
+  return rc;
+}
+
+// A re-sort that the next push carries: the state was sorted a few pushes ago
+// (the run tables of that order exist and the direct pass would be taken).
+bool sort64_fusable(const emg_ctx *c) {
+  return c->moves_after_sort <= kResortDirect && c->t64.x && c->sort_rec && c->runs_valid;
+}
+
+int sort64_rank(emg_ctx *c) {
+  timer_begin(c, kTSort);
+  const int rc = sort64_count(c);
+  timer_end(c, kTSort);
+  return rc;
+}
+
+SortDest sort64_dest(const emg_ctx *c) {
+  return SortDest{c->t64, c->sort_start, rank_buf(c), c->ncell};
+}
+
+int sort64_commit(emg_ctx *c) {
+  timer_begin(c, kTSort);
+  const int rc = sort64_swap_runs(c);
+  timer_end(c, kTSort);
+  return rc;
 }
 
 int init_device_into(emg_ctx *c, State64 &st, const emg_device_species *spec, int nspec,

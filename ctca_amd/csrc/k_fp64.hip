@@ -407,16 +407,23 @@ __global__ __launch_bounds__(kBlock) void k_deposit64(State64 s, SpeciesTab sp, 
 // The window is shaped from the run's positions before the move (read by
 // every thread before any is written: the barrier after zeroing it); the
 // particles that move out of it add to rho directly.
-__global__ __launch_bounds__(kBlock) void k_push_dep64(State64 s, SpeciesTab sp,
+// RESORT: the moved particle (and its pid) goes to its sorted slot of dst.t
+// instead of back to its own slot; the arithmetic is the same.  Four waves
+// per SIMD (at most 128 VGPRs) for both forms: the plain one needs 122, the
+// re-sorting one would take 132 and drop to three.
+template <bool RESORT>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_push_dep64(State64 s, SpeciesTab sp,
                                                        const double *__restrict__ node, Grid g,
                                                        int ustep, double *__restrict__ rho,
                                                        int64_t ncell, const Run *__restrict__ runs,
-                                                       int *flag) {
+                                                       int *flag, SortDest dst) {
   __shared__ unsigned long long win[kWinCap];
   int is;
   int64_t beg, end;
   if (!block_run(sp, runs, kBlock * kDepPer, is, beg, end)) return;  // uniform over the block
   const int64_t off = sp.off[is];
+  const int64_t key0 = (int64_t)is * dst.nkey;
+  const int64_t base = RESORT ? off - dst.start[key0] : 0;
   Win W;
   run_window(s, off + beg, off + end - 1, g, kWinCap, kWinX, kWinY, kWinZ, W.wx, W.wy, W.wz, W.ox,
              W.oy, W.oz);
@@ -430,6 +437,12 @@ __global__ __launch_bounds__(kBlock) void k_push_dep64(State64 s, SpeciesTab sp,
     const int64_t m = off + mm;
     double x = s.x[m], y = s.y[m], z = s.z[m];
     double vx = s.vx[m], vy = s.vy[m], vz = s.vz[m];
+    int64_t d = m;
+    int32_t pid = 0;
+    if (RESORT) {
+      pid = s.pid[m];
+      d = base + dst.start[key0 + slab_key_xyz(x, y, z, g)] + dst.rank[m];
+    }
     domain_guard(x, y, z, g, flag);
     int i = (int)x, j = (int)y, k = (int)z;  // int(), psolve2.F90:158-160
     Cic w = cic(x, y, z, i, j, k);
@@ -441,12 +454,14 @@ __global__ __launch_bounds__(kBlock) void k_push_dep64(State64 s, SpeciesTab sp,
     y = y + ymove;
     z = z + zmove;
     wrap_periodic(x, y, z, g, flag);
-    s.x[m] = x;
-    s.y[m] = y;
-    s.z[m] = z;
-    s.vx[m] = vx;
-    s.vy[m] = vy;
-    s.vz[m] = vz;
+    const State64 &o = RESORT ? dst.t : s;
+    o.x[d] = x;
+    o.y[d] = y;
+    o.z[d] = z;
+    o.vx[d] = vx;
+    o.vy[d] = vy;
+    o.vz[d] = vz;
+    if (RESORT) o.pid[d] = pid;
     domain_guard(x, y, z, g, flag);
     deposit_win(win, W, r, g, x, y, z, qc);
   }
@@ -503,17 +518,21 @@ constexpr double kEmFix = 281474976710656.0;          // 2^48
 
 __device__ __forceinline__ long long em_fix(double v) { return __double2ll_rn(v * kEmFix); }
 
+// RESORT as in k_push_dep64.
+template <bool RESORT>
 __global__ __launch_bounds__(kBlock) void k_push_em64(State64 s, SpeciesTab sp,
                                                       const double *__restrict__ node, Grid g,
                                                       int ustep, double *__restrict__ jacc,
                                                       int64_t ncell, const Run *__restrict__ runs,
-                                                      int *flag) {
+                                                      int *flag, SortDest dst) {
   constexpr int WN = kEmCap;
   __shared__ unsigned long long win[3 * WN];
   int is;
   int64_t beg, end;
   if (!block_run(sp, runs, kBlock * kEmPer, is, beg, end)) return;  // uniform over the block
   const int64_t off = sp.off[is];
+  const int64_t key0 = (int64_t)is * dst.nkey;
+  const int64_t base = RESORT ? off - dst.start[key0] : 0;
   // window: nodes o .. o + w - 1 per axis (w <= n, so no node appears twice)
   int wx, wy, wz, ox, oy, oz;
   run_window(s, off + beg, off + end - 1, g, WN, kEmWinX, kEmWinY, kEmWinZ, wx, wy, wz, ox, oy,
@@ -531,6 +550,12 @@ __global__ __launch_bounds__(kBlock) void k_push_em64(State64 s, SpeciesTab sp,
     const int64_t m = off + mm;
     double x = s.x[m], y = s.y[m], z = s.z[m];
     double vx = s.vx[m], vy = s.vy[m], vz = s.vz[m];
+    int64_t d = m;
+    int32_t pid = 0;
+    if (RESORT) {
+      pid = s.pid[m];
+      d = base + dst.start[key0 + slab_key_xyz(x, y, z, g)] + dst.rank[m];
+    }
     domain_guard(x, y, z, g, flag);
     double xlb = x, ylb = y, zlb = z;
     int ib = (int)floor(xlb), jb = (int)floor(ylb), kb = (int)floor(zlb);  // :170-172
@@ -550,12 +575,14 @@ __global__ __launch_bounds__(kBlock) void k_push_em64(State64 s, SpeciesTab sp,
     Relay rz = relay_axis(z, zlb, zb, kb, g.nz, g.slz, inside, pz);
     if (!(px >= 0.0 && px <= g.slx && py >= 0.0 && py <= g.sly && pz >= 0.0 && pz <= g.slz))
       atomicOr(flag, kFlagFar);
-    s.x[m] = px;
-    s.y[m] = py;
-    s.z[m] = pz;
-    s.vx[m] = vx;
-    s.vy[m] = vy;
-    s.vz[m] = vz;
+    const State64 &o = RESORT ? dst.t : s;
+    o.x[d] = px;
+    o.y[d] = py;
+    o.z[d] = pz;
+    o.vx[d] = vx;
+    o.vy[d] = vy;
+    o.vz[d] = vz;
+    if (RESORT) o.pid[d] = pid;
     // :717-757
     double xd1 = (rx.r + xlb) * 0.5 - xb, yd1 = (ry.r + ylb) * 0.5 - yb, zd1 = (rz.r + zlb) * 0.5 - zb;
     double vvx1 = (rx.r - xlb) * rustep, vvy1 = (ry.r - ylb) * rustep, vvz1 = (rz.r - zlb) * rustep;
@@ -704,13 +731,18 @@ static dim3 block_grid(const emg_ctx *c, int per, int which, const Run *&runs) {
   return dim3((unsigned)((mx + per - 1) / per), c->sp.nspec);
 }
 
-void launch_push_es64(emg_ctx *c, int ustep) {
+void launch_push_es64(emg_ctx *c, int ustep, bool resort) {
   launch_zero(c->rho64, sizeof(double) * c->ncell * c->sp.nspec, c->stream);
   timer_begin(c, kTPush);
   const Run *runs;
   const dim3 grid = block_grid(c, kBlock * kDepPer, 0, runs);
-  hipLaunchKernelGGL(k_push_dep64, grid, dim3(kBlock), 0, c->stream, c->s64, c->sp, c->node64,
-                     c->g, ustep, c->rho64, c->ncell, runs, c->dflag);
+  if (resort)
+    hipLaunchKernelGGL(k_push_dep64<true>, grid, dim3(kBlock), 0, c->stream, c->s64, c->sp,
+                       c->node64, c->g, ustep, c->rho64, c->ncell, runs, c->dflag,
+                       sort64_dest(c));
+  else
+    hipLaunchKernelGGL(k_push_dep64<false>, grid, dim3(kBlock), 0, c->stream, c->s64, c->sp,
+                       c->node64, c->g, ustep, c->rho64, c->ncell, runs, c->dflag, SortDest{});
   timer_end(c, kTPush);
 }
 
@@ -724,13 +756,17 @@ void launch_deposit64(emg_ctx *c) {
   timer_end(c, kTDeposit);
 }
 
-void launch_push_em64(emg_ctx *c, int ustep) {
+void launch_push_em64(emg_ctx *c, int ustep, bool resort) {
   launch_zero(c->j64, sizeof(double) * 3 * c->ncell * c->sp.nspec, c->stream);
   timer_begin(c, kTPush);
   const Run *runs;
   const dim3 grid = block_grid(c, kBlock * kEmPer, 1, runs);
-  hipLaunchKernelGGL(k_push_em64, grid, dim3(kBlock), 0, c->stream, c->s64, c->sp, c->node64,
-                     c->g, ustep, c->j64, c->ncell, runs, c->dflag);
+  if (resort)
+    hipLaunchKernelGGL(k_push_em64<true>, grid, dim3(kBlock), 0, c->stream, c->s64, c->sp,
+                       c->node64, c->g, ustep, c->j64, c->ncell, runs, c->dflag, sort64_dest(c));
+  else
+    hipLaunchKernelGGL(k_push_em64<false>, grid, dim3(kBlock), 0, c->stream, c->s64, c->sp,
+                       c->node64, c->g, ustep, c->j64, c->ncell, runs, c->dflag, SortDest{});
   timer_end(c, kTPush);
 }
 

@@ -554,6 +554,54 @@ def test_sort_every_only_permutes():
         ctx_for(n, s["q"], s["qm"], s["rm"], s["counts"], sort_every=-1)
 
 
+def test_resorts_carried_by_pushes_only_permute():
+    """sort_every = 2 over ES pushes, a positn and EM pushes: every re-sort
+    after the first is carried by the push that follows it (the push writes
+    each particle to its slot in the new order), except the one before the
+    positn, which sorts on its own.  Positions and velocities stay
+    bit-identical to a run that never sorts (matched by pid), rho and J agree
+    to round-off after every push, and each species keeps its own particles."""
+    s = SMALL
+    n = s["n"]
+    rng = np.random.default_rng(7)
+    eb = periodic_eb(n, rng, 0.05, 0.02)
+    p = small_state()
+    # sorts before pushes 1 (a sort of its own: no earlier order), 3 (carried),
+    # 5 (the positn: its own), 7 and 9 (carried by zigzag pushes)
+    seq = ["es"] * 4 + ["positn"] + ["em"] * 4
+    out = {}
+    for every in (0, 2):
+        with ctx_for(n, s["q"], s["qm"], s["rm"], s["counts"], b0=(0.0, 0.0, 0.03),
+                     sort_every=every) as ctx:
+            ctx.upload(p, s["counts"])
+            ctx.set_fields(eb)
+            grids, states = [], []
+            for op in seq:
+                if op == "es":
+                    ctx.push_es(2)
+                    grids.append(ctx.rho())
+                elif op == "em":
+                    ctx.push_em(2)
+                    grids.append(ctx.current())
+                else:
+                    ctx.positn(1)
+                    grids.append(None)
+                states.append(ctx.download()[0])
+        out[every] = (grids, states)
+    for k, op in enumerate(seq):
+        a, b = out[0][1][k], out[2][1][k]
+        ap, bp = by_pid(a), by_pid(b)
+        for f in FIELDS:
+            assert np.array_equal(ap[f], bp[f]), (k, op, f)
+        off = 0
+        for c in s["counts"]:
+            assert np.array_equal(np.sort(a[off:off + c]["pid"]), np.sort(b[off:off + c]["pid"]))
+            off += c
+        if op != "positn":
+            ga, gb = out[0][0][k], out[2][0][k]
+            assert np.abs(gb - ga).max() <= 1e-12 * np.abs(ga).max(), (k, op)
+
+
 def test_sorted_runs_on_a_nonuniform_grid():
     """After a sort the FP64 kernels take their blocks from the sort's run
     table (chunks of consecutive cells, each cut into runs of nearly equal
