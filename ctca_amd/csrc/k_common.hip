@@ -686,9 +686,8 @@ int32_t *rank_buf(const emg_ctx *c) {
   return reinterpret_cast<int32_t *>(static_cast<char *>(c->sort_rec) + 64 * (size_t)c->cap_total);
 }
 
-// the sort's keys and ranks: count each key's particles (a particle's rank
-// is the count it finds) and scan the counts into each key's start
-int sort64_count(emg_ctx *c) {
+// the sort's buffers (once per context; before the timed region)
+int sort64_alloc(emg_ctx *c) {
   int64_t nkey = c->ncell;
   int64_t len = nkey * c->sp.nspec;
   if (c->sort_len < len) {
@@ -715,8 +714,10 @@ int sort64_count(emg_ctx *c) {
   if (!c->t64.x) {
     void *p[7];
     for (int q = 0; q < 7; q++)
-      if (hipMalloc(&p[q], (q < 6 ? sizeof(double) : sizeof(int32_t)) * c->cap_total) != hipSuccess)
+      if (hipMalloc(&p[q], (q < 6 ? sizeof(double) : sizeof(int32_t)) * c->cap_total) != hipSuccess) {
+        for (int r = 0; r < q; r++) (void)hipFree(p[r]);
         return set_error(EMG_ERR_NOMEM, "sort buffers");
+      }
     c->t64 = State64{(double *)p[0], (double *)p[1], (double *)p[2], (double *)p[3],
                      (double *)p[4], (double *)p[5], (int32_t *)p[6]};
   }
@@ -724,6 +725,14 @@ int sort64_count(emg_ctx *c) {
     c->sort_rec = nullptr;
     return set_error(EMG_ERR_NOMEM, "sort records");
   }
+  return EMG_OK;
+}
+
+// the sort's keys and ranks: count each key's particles (a particle's rank
+// is the count it finds) and scan the counts into each key's start
+int sort64_count(emg_ctx *c) {
+  const int64_t nkey = c->ncell;
+  const int64_t len = nkey * c->sp.nspec;
   launch_zero(c->sort_count, sizeof(int32_t) * len, c->stream);
   hipLaunchKernelGGL(k_sort_hist, species_grid(c->sp), dim3(kBlock), 0, c->stream, c->s64, c->sp,
                      c->g, nkey, c->sort_count, rank_buf(c));
@@ -801,8 +810,10 @@ int sort64_swap_runs(emg_ctx *c) {
 }  // namespace
 
 int sort64(emg_ctx *c) {
+  int rc = sort64_alloc(c);
+  if (rc) return rc;
   timer_begin(c, kTSort);
-  int rc = sort64_count(c);
+  rc = sort64_count(c);
   if (!rc) {
     const int64_t nkey = c->ncell;
     const dim3 grid = species_grid(c->sp);
@@ -828,8 +839,10 @@ bool sort64_fusable(const emg_ctx *c) {
 }
 
 int sort64_rank(emg_ctx *c) {
+  int rc = sort64_alloc(c);
+  if (rc) return rc;
   timer_begin(c, kTSort);
-  const int rc = sort64_count(c);
+  rc = sort64_count(c);
   timer_end(c, kTSort);
   return rc;
 }
