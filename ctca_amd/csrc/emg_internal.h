@@ -118,6 +118,33 @@ __host__ __device__ inline int64_t slab_key_xyz(double x, double y, double z, co
   return ((int64_t)k * g.ny + j) * g.nx + i;
 }
 
+// The active lanes of a wave grouped by equal key, one ballot per distinct
+// key (no memory access).  Every lane of the wave must call it (the loop is
+// uniform).  lead: the group's first lane; inner: the lane's rank in its
+// group, in lane order; size: the group's size, on the lead lane.  A caller
+// then sends one atomic per group (the lead lane's) instead of one per lane.
+__device__ inline void wave_groups(int64_t key, bool act, int &lead, int &inner, int &size) {
+  const int lane = __lane_id();
+  unsigned long long todo = __ballot(act);
+  lead = lane;
+  inner = 0;
+  size = 0;
+  while (todo) {
+    const int l = __builtin_amdgcn_readfirstlane(__ffsll((long long)todo) - 1);
+    const int64_t k = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(key >> 32), l)
+                                 << 32) |
+                                (uint32_t)__builtin_amdgcn_readlane((int)key, l));
+    const unsigned long long grp = __ballot(act && key == k);
+    if (lane == l) size = __popcll(grp);
+    if ((grp >> lane) & 1ull) {
+      lead = l;
+      inner = __builtin_amdgcn_mbcnt_hi((uint32_t)(grp >> 32),
+                                        __builtin_amdgcn_mbcnt_lo((uint32_t)grp, 0));
+    }
+    todo &= ~grp;
+  }
+}
+
 // Where a push that carries a re-sort writes particle m of species is: slot
 // off[is] + start[is nkey + key] - start[is nkey] + rank[m] of t, key being
 // the cell of its position before the move (what sort64_rank counted).

@@ -348,22 +348,8 @@ __global__ __launch_bounds__(kBlock) void k_sort_hist(State64 s, SpeciesTab sp, 
   const bool act = m < sp.cnt[is];  // the whole wave takes part in the ballots
   const int lane = __lane_id();
   const int64_t key = act ? (int64_t)is * nkey + slab_key(s, m + sp.off[is], g) : -1;
-  unsigned long long todo = __ballot(act);
-  int lead = lane, size = 0, inner = 0;
-  while (todo) {  // uniform over the wave; one round per distinct key
-    const int l = __builtin_amdgcn_readfirstlane(__ffsll((long long)todo) - 1);
-    const int64_t k = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(key >> 32), l)
-                                 << 32) |
-                                (uint32_t)__builtin_amdgcn_readlane((int)key, l));
-    const unsigned long long grp = __ballot(key == k);  // k >= 0: inactive lanes never match
-    if (lane == l) size = __popcll(grp);
-    if ((grp >> lane) & 1ull) {
-      lead = l;
-      inner = __builtin_amdgcn_mbcnt_hi((uint32_t)(grp >> 32),
-                                        __builtin_amdgcn_mbcnt_lo((uint32_t)grp, 0));
-    }
-    todo &= ~grp;
-  }
+  int lead, inner, size;
+  wave_groups(key, act, lead, inner, size);
   int base = 0;
   if (act && lead == lane) base = atomicAdd(&count[key], size);
   base = __shfl(base, lead);
