@@ -5,8 +5,8 @@
   directly from where they lie under /root/reference (nothing is copied):
     lib/mtarm/mt19937arm.c     (MT19937 + the Fortran shims mt_init_/mtrand_real8_)
     lib/psort/psort-struct.c   (particle_sort_ / particle_spec_ qsort comparators)
-  Built only when /root/reference exists (not on the GPU box, which uses the
-  prebuilt file).  The Fortran path (psolve2/charge/psolve1) needs HDF5, MPI,
+  Built only where /root/reference exists; a machine without those sources
+  uses the prebuilt file.  The Fortran path (psolve2/charge/psolve1) needs HDF5, MPI,
   FFTW and unpinned fpm git dependencies and is unbuildable here.
 """
 import os
