@@ -411,15 +411,24 @@ __device__ __forceinline__ void zig_w4(float u, float w, float W[4]) {
   W[0] = 1.f - u - W[2];
 }
 
+// a current contribution rounded to an integer, halves up: floor(v + 0.5) in
+// one instruction (v_cvt_rpi_i32_f32; the compiler emits it only under
+// fast-math flags, which the rest of this file must not have)
+__device__ __forceinline__ int round_half_up(float v) {
+  int r;
+  asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(v));
+  return r;
+}
+
 // fixed point of a current contribution (kJScale per unit flux; a flux is at
 // most a few cells per unit time, far inside the int32 range)
 __device__ __forceinline__ unsigned long long jfix(float v) {
-  return (unsigned long long)(long long)(int)rintf(v * (float)kJScale);
+  return (unsigned long long)(long long)round_half_up(v * (float)kJScale);
 }
 
 // fixed point of a flux already scaled by kJScale times a weight: the same
 // value as jfix(flux * w), since scaling by a power of two is exact
-__device__ __forceinline__ int jfix_scaled(float fs, float w) { return (int)rintf(fs * w); }
+__device__ __forceinline__ int jfix_scaled(float fs, float w) { return round_half_up(fs * w); }
 
 // Zigzag current of one particle into the block's current tiles: segment 1
 // at tile cell q1, segment 2 at q2.  When both segments lie in one cell (the
@@ -450,7 +459,7 @@ __device__ __forceinline__ void zig_lds(unsigned long long *jh, int q1, int q2, 
     zig_w4(u2[c], w2[c], U);
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-      const int v = (int)rintf(fmaf(same ? f2[c] : 0.f, U[k], f1[c] * W[k]));
+      const int v = round_half_up(fmaf(same ? f2[c] : 0.f, U[k], f1[c] * W[k]));
       atomicAdd(&jh[q1 + o[c][k]], (unsigned long long)(long long)v);
     }
   }
