@@ -26,11 +26,14 @@ def eb_random(n, seed, amp=0.01):
     return oracle.fields.eb_from_phi(rng.normal(size=(n[2], n[1], n[0])) * amp)
 
 
-def run_both(n, counts, p, eb, steps, tile=None, b0=(0.0, 0.0, 0.02)):
-    """(FP64 result, FP32 result, oracle result, FP32 start) after `steps` ES pushes."""
+def run_both(n, counts, p, eb, steps, tile=None, b0=(0.0, 0.0, 0.02), sort_every=0):
+    """(FP64 result, FP32 result, oracle result, FP32 start) after `steps` ES pushes;
+    sort_every applies to the FP64 form (2: sorted before the first push, and the
+    re-sort before the third carried by that push)."""
     out = {}
     for prec in (emg.PLAIN_FP64, emg.FP32_FIXED):
-        kw = dict(track_pid=True, tile=tile) if prec == emg.FP32_FIXED else {}
+        kw = (dict(track_pid=True, tile=tile) if prec == emg.FP32_FIXED
+              else dict(sort_every=sort_every))
         with emg.Context(n, Q, QM, RM, counts, precision=prec, b0=b0, **kw) as ctx:
             ctx.upload(p, counts)
             start, _ = ctx.download()
@@ -71,12 +74,13 @@ def check(n, counts, res, o, rhodg, bitwise):
         assert np.abs(rho[k] - ok).max() <= (1e-12 if bitwise else 1e-5) * scale, k
 
 
-def test_empty_species():
+@pytest.mark.parametrize("sort_every", [0, 2])
+def test_empty_species(sort_every):
     n = (16, 8, 8)
     counts = (20000, 0)
     p = particles(n, (20000, 1), (0.1, 0.01), 3)[:20000]
     eb = eb_random(n, 1)
-    res = run_both(n, counts, p, eb, 3, tile=8)
+    res = run_both(n, counts, p, eb, 3, tile=8, sort_every=sort_every)
     o, rhodg = oracle_run(n, counts, res[emg.PLAIN_FP64][2], eb, 3)
     check(n, counts, res[emg.PLAIN_FP64], o, rhodg, True)
     o, rhodg = oracle_run(n, counts, res[emg.FP32_FIXED][2], eb, 3)
@@ -84,15 +88,17 @@ def test_empty_species():
     assert not np.any(res[emg.FP32_FIXED][1][1])   # the empty species deposits nothing
 
 
-def test_one_tile_per_axis():
+@pytest.mark.parametrize("sort_every", [0, 2])
+def test_one_tile_per_axis(sort_every):
     """n = 4^3 with 4^3-cell tiles: each of the 26 neighbouring tiles is the
     tile itself, so every mover leaves through its own outbox and is pulled
-    back by its own block."""
+    back by its own block.  With sort_every = 2 the FP64 form's runs hold
+    over a hundred particles per cell and a re-sort is carried by a push."""
     n = (4, 4, 4)
     counts = (8000, 4000)
     p = particles(n, counts, (0.3, 0.05), 5)
     eb = eb_random(n, 2)
-    res = run_both(n, counts, p, eb, 4, tile=4)
+    res = run_both(n, counts, p, eb, 4, tile=4, sort_every=sort_every)
     o, rhodg = oracle_run(n, counts, res[emg.FP32_FIXED][2], eb, 4)
     check(n, counts, res[emg.FP32_FIXED], o, rhodg, False)
     o, rhodg = oracle_run(n, counts, res[emg.PLAIN_FP64][2], eb, 4)
