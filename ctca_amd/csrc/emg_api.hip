@@ -806,7 +806,8 @@ int emg_push_es(emg_ctx *c, int32_t ustep) {
   if ((rc = sort_if_due(c, &carry))) return rc;
   if (c->cfg.precision == EMG_PLAIN_FP64) {
     launch_push_es64(c, ustep, carry);  // push + charge at the new positions
-    if (carry && (rc = emg::sort64_commit(c))) return rc;
+    // the carried re-sort's buffers are swapped even if its run tables fail
+    if (carry) rc = emg::sort64_commit(c);
     if (c->moves_after_sort < emg::kUnsorted) c->moves_after_sort++;
     c->rho_valid = true;
     c->faces_valid = false;
@@ -818,6 +819,7 @@ int emg_push_es(emg_ctx *c, int32_t ustep) {
     c->ke_valid = true;  // fused partials of the new velocities
   }
   c->j_valid = false;
+  if (rc) return rc;
   return hip_check(hipGetLastError(), "push launch");
 }
 
@@ -926,7 +928,8 @@ int emg_push_em(emg_ctx *c, int32_t ustep) {
   if ((rc = sort_if_due(c, &carry))) return rc;
   if (c->cfg.precision == EMG_PLAIN_FP64) {
     launch_push_em64(c, ustep, carry);
-    if (carry && (rc = emg::sort64_commit(c))) return rc;
+    // the carried re-sort's buffers are swapped even if its run tables fail
+    if (carry) rc = emg::sort64_commit(c);
     if (c->moves_after_sort < emg::kUnsorted) c->moves_after_sort++;
     c->ke_valid = false;
   } else {
@@ -936,6 +939,7 @@ int emg_push_em(emg_ctx *c, int32_t ustep) {
   c->j_valid = true;
   c->rho_valid = false;
   c->faces_valid = false;
+  if (rc) return rc;
   return hip_check(hipGetLastError(), "push_em launch");
 }
 
