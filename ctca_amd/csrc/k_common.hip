@@ -334,25 +334,26 @@ __device__ __forceinline__ int64_t slab_key(const State64 &s, int64_t m, const G
 }
 
 // count the particles of each key; the count a particle finds is its rank
-// among the key's particles (kept for the scatter).  The input is the last
-// sort's order moved by a few pushes: a wave holds a few distinct keys, the
-// particles that crossed a face interleaved with those that stayed.  The wave
-// groups its lanes by key (one ballot per distinct key, no memory access),
-// then each group's first lane adds the group's size with one atomic -- all
-// groups' atomics in one instruction -- and the lanes rank themselves inside
-// their group from the count it returns.
+// among the key's particles (kept for the scatter).  A re-sort's input is the
+// last sort's order moved by a few pushes: a wave holds a few distinct keys,
+// the particles that crossed a face interleaved with those that stayed.  With
+// `grouped` the wave groups its lanes by key (one ballot per distinct key, no
+// memory access), then each group's first lane adds the group's size with one
+// atomic -- all groups' atomics in one instruction -- and the lanes rank
+// themselves inside their group from the count it returns.  An input in no
+// known order (up to 64 keys per wave) takes one atomic per lane.
 __global__ __launch_bounds__(kBlock) void k_sort_hist(State64 s, SpeciesTab sp, Grid g, int64_t nkey,
-                                                      int32_t *count, int32_t *rank) {
+                                                      int32_t *count, int32_t *rank, bool grouped) {
   const int is = blockIdx.y;
   const int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const bool act = m < sp.cnt[is];  // the whole wave takes part in the ballots
   const int lane = __lane_id();
   const int64_t key = act ? (int64_t)is * nkey + slab_key(s, m + sp.off[is], g) : -1;
-  int lead, inner, size;
-  wave_groups(key, act, lead, inner, size);
+  int lead = lane, inner = 0, size = 1;
+  if (grouped) wave_groups(key, act, lead, inner, size);  // uniform over the grid
   int base = 0;
   if (act && lead == lane) base = atomicAdd(&count[key], size);
-  base = __shfl(base, lead);
+  if (grouped) base = __shfl(base, lead);
   if (act) rank[m + sp.off[is]] = base + inner;
 }
 
@@ -721,7 +722,7 @@ int sort64_count(emg_ctx *c) {
   const int64_t len = nkey * c->sp.nspec;
   launch_zero(c->sort_count, sizeof(int32_t) * len, c->stream);
   hipLaunchKernelGGL(k_sort_hist, species_grid(c->sp), dim3(kBlock), 0, c->stream, c->s64, c->sp,
-                     c->g, nkey, c->sort_count, rank_buf(c));
+                     c->g, nkey, c->sort_count, rank_buf(c), c->moves_after_sort <= kResortDirect);
   exclusive_scan_i32(c->sort_count, c->sort_start, len, c->scan_tmp, c->stream);
   return hip_check(hipGetLastError(), "sort");
 }
