@@ -412,11 +412,10 @@ __global__ __launch_bounds__(kBlock) void k_deposit64(State64 s, SpeciesTab sp, 
 // per SIMD (at most 128 VGPRs) for both forms: the plain one needs 122, the
 // re-sorting one would take 132 and drop to three.
 template <bool RESORT>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_push_dep64(State64 s, SpeciesTab sp,
-                                                       const double *__restrict__ node, Grid g,
-                                                       int ustep, double *__restrict__ rho,
-                                                       int64_t ncell, const Run *__restrict__ runs,
-                                                       int *flag, SortDest dst) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
+void k_push_dep64(State64 s, SpeciesTab sp, const double *__restrict__ node, Grid g, int ustep,
+                  double *__restrict__ rho, int64_t ncell, const Run *__restrict__ runs, int *flag,
+                  SortDest dst) {
   __shared__ unsigned long long win[kWinCap];
   int is;
   int64_t beg, end;
