@@ -554,25 +554,28 @@ def test_sort_every_only_permutes():
         ctx_for(n, s["q"], s["qm"], s["rm"], s["counts"], sort_every=-1)
 
 
-def test_resorts_carried_by_pushes_only_permute():
-    """sort_every = 2 over ES pushes, a positn and EM pushes: every re-sort
-    after the first is carried by the push that follows it (the push writes
-    each particle to its slot in the new order), except the one before the
-    positn, which sorts on its own.  Positions and velocities stay
-    bit-identical to a run that never sorts (matched by pid), rho and J agree
-    to round-off after every push, and each species keeps its own particles."""
+@pytest.mark.parametrize("every", [1, 2])
+def test_resorts_carried_by_pushes_only_permute(every):
+    """sort_every = 1 or 2 over ES pushes, a positn and EM pushes: every
+    re-sort after the first is carried by the push that follows it (the push
+    writes each particle to its slot in the new order; with 1, each carried
+    push runs on the order and run tables the previous one left), except the
+    one before the positn, which sorts on its own.  Positions and velocities
+    stay bit-identical to a run that never sorts (matched by pid), rho and J
+    agree to round-off after every push, and each species keeps its own
+    particles."""
     s = SMALL
     n = s["n"]
     rng = np.random.default_rng(7)
     eb = periodic_eb(n, rng, 0.05, 0.02)
     p = small_state()
-    # sorts before pushes 1 (a sort of its own: no earlier order), 3 (carried),
-    # 5 (the positn: its own), 7 and 9 (carried by zigzag pushes)
+    # with 2: sorts before pushes 1 (a sort of its own: no earlier order), 3
+    # (carried), 5 (the positn: its own), 7 and 9 (carried by zigzag pushes)
     seq = ["es"] * 4 + ["positn"] + ["em"] * 4
     out = {}
-    for every in (0, 2):
+    for se in (0, every):
         with ctx_for(n, s["q"], s["qm"], s["rm"], s["counts"], b0=(0.0, 0.0, 0.03),
-                     sort_every=every) as ctx:
+                     sort_every=se) as ctx:
             ctx.upload(p, s["counts"])
             ctx.set_fields(eb)
             grids, states = [], []
@@ -587,9 +590,9 @@ def test_resorts_carried_by_pushes_only_permute():
                     ctx.positn(1)
                     grids.append(None)
                 states.append(ctx.download()[0])
-        out[every] = (grids, states)
+        out[se] = (grids, states)
     for k, op in enumerate(seq):
-        a, b = out[0][1][k], out[2][1][k]
+        a, b = out[0][1][k], out[every][1][k]
         ap, bp = by_pid(a), by_pid(b)
         for f in FIELDS:
             assert np.array_equal(ap[f], bp[f]), (k, op, f)
@@ -598,7 +601,7 @@ def test_resorts_carried_by_pushes_only_permute():
             assert np.array_equal(np.sort(a[off:off + c]["pid"]), np.sort(b[off:off + c]["pid"]))
             off += c
         if op != "positn":
-            ga, gb = out[0][0][k], out[2][0][k]
+            ga, gb = out[0][0][k], out[every][0][k]
             assert np.abs(gb - ga).max() <= 1e-12 * np.abs(ga).max(), (k, op)
 
 
